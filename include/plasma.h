@@ -1,0 +1,21 @@
+/*
+ * plasma.h -- umbrella header of the MI355X implementation of PLASMA's
+ * Cholesky solve path (reference include/plasma.h).  Only the routines of
+ * that path are provided: plasma_{d,s}{potrf,potrs,posv}, plasma_dsposv,
+ * their plasma_omp_* tile forms, the layout translations and the FP64/FP32
+ * conversions.
+ */
+#ifndef PLASMA_H
+#define PLASMA_H
+
+#include "plasma_types.h"
+#include "plasma_error.h"
+#include "plasma_async.h"
+#include "plasma_descriptor.h"
+#include "plasma_context.h"
+#include "plasma_d.h"
+#include "plasma_s.h"
+#include "plasma_ds.h"
+#include "plasma_mi355x.h"
+
+#endif /* PLASMA_H */

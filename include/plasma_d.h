@@ -1,0 +1,69 @@
+/*
+ * plasma_d.h -- double precision Cholesky solve path.
+ *
+ * Signatures are the reference's d-precision forms (generated from
+ * include/plasma_z.h:272-292, :398-430, :631-646, :667-671).
+ *
+ * LAPACK-layout routines (plasma_dpotrf, plasma_dpotrs, plasma_dposv) take
+ * column-major arrays that may live in host memory or in device memory of
+ * the library's GPU; the library detects which.  Host arrays are staged
+ * through HBM; device arrays are converted in place on the GPU.  The call
+ * returns when all work has finished, like the reference's implicit
+ * barrier.
+ *
+ * Tile routines (plasma_omp_*) take descriptors whose storage is device
+ * memory and enqueue their work on the library's streams; completion and
+ * the final sequence->status come from plasma_sequence_sync().  The
+ * pA arguments of the layout-translation routines may be host or device
+ * pointers; for host pointers those two routines complete before they
+ * return.
+ *
+ * Return codes: 0 success; -i: the i-th argument had an illegal value;
+ * i > 0: the leading minor of order i is not positive definite (global,
+ * 1-based, as LAPACK dpotrf); negative PlasmaError* codes for library
+ * errors.
+ */
+#ifndef PLASMA_D_H
+#define PLASMA_D_H
+
+#include "plasma_async.h"
+#include "plasma_descriptor.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* reference compute/zpotrf.c:74 */
+int plasma_dpotrf(plasma_enum_t uplo, int n, double *pA, int lda);
+/* reference compute/zpotrs.c:77 */
+int plasma_dpotrs(plasma_enum_t uplo, int n, int nrhs,
+                  double *pA, int lda, double *pB, int ldb);
+/* reference compute/zposv.c:90 */
+int plasma_dposv(plasma_enum_t uplo, int n, int nrhs,
+                 double *pA, int lda, double *pB, int ldb);
+
+/* reference compute/zpotrf.c:206 */
+void plasma_omp_dpotrf(plasma_enum_t uplo, plasma_desc_t A,
+                       plasma_sequence_t *sequence, plasma_request_t *request);
+/* reference compute/zpotrs.c:226 */
+void plasma_omp_dpotrs(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
+                       plasma_sequence_t *sequence, plasma_request_t *request);
+/* reference compute/zposv.c:244 */
+void plasma_omp_dposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
+                      plasma_sequence_t *sequence, plasma_request_t *request);
+
+/* reference compute/zge2desc.c:27, zdesc2ge.c, ztr2desc.c, zdesc2tr.c */
+void plasma_omp_dge2desc(double *pA, int lda, plasma_desc_t A,
+                         plasma_sequence_t *sequence, plasma_request_t *request);
+void plasma_omp_ddesc2ge(plasma_desc_t A, double *pA, int lda,
+                         plasma_sequence_t *sequence, plasma_request_t *request);
+void plasma_omp_dtr2desc(double *pA, int lda, plasma_desc_t A,
+                         plasma_sequence_t *sequence, plasma_request_t *request);
+void plasma_omp_ddesc2tr(plasma_desc_t A, double *pA, int lda,
+                         plasma_sequence_t *sequence, plasma_request_t *request);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PLASMA_D_H */

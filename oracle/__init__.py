@@ -1,0 +1,144 @@
+"""Python access to the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py as the checker; the product never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+LOWER, UPPER = 122, 121
+BUILTIN, MKL = 0, 1
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-C", HERE, "-B", "-s"], check=True)
+    return LIB_PATH
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        lib = ctypes.CDLL(LIB_PATH)
+        i, p, d = ctypes.c_int, ctypes.c_void_p, ctypes.c_double
+        pi = ctypes.POINTER(ctypes.c_int)
+        sig = {
+            "ora_set_blas": (i, [i]), "ora_get_blas": (i, []),
+            "ora_set_threads": (None, [i]), "ora_get_threads": (i, []),
+            "ora_plant_mistake": (None, [i]),
+            "ora_dpotrf": (i, [i, i, p, i, i]),
+            "ora_spotrf": (i, [i, i, p, i, i]),
+            "ora_dpotrs": (i, [i, i, i, p, i, p, i, i]),
+            "ora_dposv": (i, [i, i, i, p, i, p, i, i]),
+            "ora_sposv": (i, [i, i, i, p, i, p, i, i]),
+            "ora_dsposv": (i, [i, i, i, p, i, p, i, p, i, pi, i]),
+            "ora_dlag2s": (i, [i, i, p, i, p, i]),
+            "ora_slag2d": (None, [i, i, p, i, p, i]),
+            "ora_tile_storage_elems": (ctypes.c_long, [i, i, i, i]),
+            "ora_lapack2tile": (None, [i, i, i, p, i, p, i]),
+            "ora_tile2lapack": (None, [i, i, i, p, p, i, i]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(lib, name)
+            f.restype, f.argtypes = res, args
+        _lib = lib
+    return _lib
+
+
+def _f(a, dtype):
+    a = np.asarray(a, dtype=dtype)
+    return a if a.flags.f_contiguous else np.asfortranarray(a)
+
+
+def _ld(a):
+    """Leading dimension of a column-major 2-D array (or a 1-D column)."""
+    if a.ndim == 1:
+        return max(1, a.shape[0])
+    return max(1, a.strides[1] // a.itemsize)
+
+
+def set_blas(backend):
+    return load().ora_set_blas(backend)
+
+
+def set_threads(n):
+    load().ora_set_threads(n)
+
+
+def potrf(uplo, a, nb, dtype=np.float64):
+    """Returns (factor array, info); the input is not modified."""
+    a = _f(a, dtype).copy(order="F")
+    fn = load().ora_dpotrf if dtype == np.float64 else load().ora_spotrf
+    info = fn(uplo, a.shape[0], a.ctypes.data, _ld(a), nb)
+    return a, info
+
+
+def posv(uplo, a, b, nb, dtype=np.float64):
+    a = _f(a, dtype).copy(order="F")
+    b = _f(b, dtype).copy(order="F")
+    fn = load().ora_dposv if dtype == np.float64 else load().ora_sposv
+    info = fn(uplo, a.shape[0], b.shape[1], a.ctypes.data, _ld(a),
+              b.ctypes.data, _ld(b), nb)
+    return a, b, info
+
+
+def potrs(uplo, l, b, nb):
+    l = _f(l, np.float64)
+    b = _f(b, np.float64).copy(order="F")
+    info = load().ora_dpotrs(uplo, l.shape[0], b.shape[1], l.ctypes.data,
+                             _ld(l), b.ctypes.data, _ld(b), nb)
+    return b, info
+
+
+def dsposv(uplo, a, b, nb):
+    a = _f(a, np.float64)
+    b = _f(b, np.float64)
+    x = np.zeros_like(b, order="F")
+    it = ctypes.c_int(0)
+    info = load().ora_dsposv(uplo, a.shape[0], b.shape[1], a.ctypes.data,
+                             _ld(a), b.ctypes.data, _ld(b), x.ctypes.data,
+                             _ld(x), ctypes.byref(it), nb)
+    return x, it.value, info
+
+
+def dlag2s(a):
+    a = _f(a, np.float64)
+    s = np.zeros(a.shape, dtype=np.float32, order="F")
+    info = load().ora_dlag2s(a.shape[0], a.shape[1], a.ctypes.data, _ld(a),
+                             s.ctypes.data, _ld(s))
+    return s, info
+
+
+def slag2d(s):
+    s = _f(s, np.float32)
+    a = np.zeros(s.shape, dtype=np.float64, order="F")
+    load().ora_slag2d(s.shape[0], s.shape[1], s.ctypes.data, _ld(s),
+                      a.ctypes.data, _ld(a))
+    return a
+
+
+def lapack2tile(uplo, a, nb):
+    a = _f(a, np.float64)
+    m, n = a.shape
+    t = np.zeros(load().ora_tile_storage_elems(uplo, m, n, nb))
+    load().ora_lapack2tile(uplo, m, n, a.ctypes.data, _ld(a), t.ctypes.data, nb)
+    return t
+
+
+def tile2lapack(uplo, tiles, m, n, nb, init=None):
+    a = np.zeros((m, n), order="F") if init is None else _f(init, np.float64).copy(order="F")
+    t = np.ascontiguousarray(tiles, dtype=np.float64)
+    load().ora_tile2lapack(uplo, m, n, t.ctypes.data, a.ctypes.data, _ld(a), nb)
+    return a
+
+
+def plant_mistake(which):
+    load().ora_plant_mistake(which)

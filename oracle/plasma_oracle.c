@@ -1,0 +1,451 @@
+/*
+ * plasma_oracle.c -- CPU restatement of PLASMA's tile Cholesky solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the correctness oracle and CPU baseline for the
+ * MI355X library.  Only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg load it.
+ *
+ * Layout: the reference's tile descriptor (include/plasma_descriptor.h:
+ * general tiles column by column then the edge regions A21 / A12 / A22,
+ * triangular descriptors storing one triangle of tiles).  Algorithms: one
+ * OpenMP task per tile operation, depend clauses on the tiles, as in
+ * compute/pzpotrf.c, pztrsm.c and zcposv.c.
+ */
+#define _GNU_SOURCE
+#include <dlfcn.h>
+#include <float.h>
+#include <math.h>
+#include <omp.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "plasma_oracle.h"
+
+enum { ORA_NOTRANS = 111, ORA_TRANS = 112, ORA_LEFT = 141, ORA_RIGHT = 142 };
+
+static int ora_backend = ORA_BLAS_BUILTIN;
+static int ora_threads = 0;
+static int ora_planted = 0;
+
+void ora_plant_mistake(int which) { ora_planted = which; }
+void ora_set_threads(int n) { ora_threads = n; }
+int ora_get_threads(void) { return ora_threads > 0 ? ora_threads : omp_get_max_threads(); }
+static int ora_nthreads(void) { return ora_get_threads(); }
+
+static void ora_fail(volatile int *status, int info)
+{
+    #pragma omp critical(ora_status)
+    {
+        if (*status == 0) *status = info;
+    }
+}
+
+/* ---------------------------------------------------------------------
+ * Vendor BLAS / LAPACK through dlopen (the reference links any vendor's
+ * CBLAS + LAPACKE; MKL 2021.4 is what this image provides).
+ * --------------------------------------------------------------------- */
+typedef void (*gemm_d_t)(int, int, int, int, int, int, double, const double *,
+                         int, const double *, int, double, double *, int);
+typedef void (*trsm_d_t)(int, int, int, int, int, int, int, double,
+                         const double *, int, double *, int);
+typedef void (*syrk_d_t)(int, int, int, int, int, double, const double *, int,
+                         double, double *, int);
+typedef void (*symm_d_t)(int, int, int, int, int, double, const double *, int,
+                         const double *, int, double, double *, int);
+typedef int (*potrf_d_t)(int, char, int, double *, int);
+typedef void (*gemm_s_t)(int, int, int, int, int, int, float, const float *,
+                         int, const float *, int, float, float *, int);
+typedef void (*trsm_s_t)(int, int, int, int, int, int, int, float,
+                         const float *, int, float *, int);
+typedef void (*syrk_s_t)(int, int, int, int, int, float, const float *, int,
+                         float, float *, int);
+typedef void (*symm_s_t)(int, int, int, int, int, float, const float *, int,
+                         const float *, int, float, float *, int);
+typedef int (*potrf_s_t)(int, char, int, float *, int);
+
+static struct {
+    gemm_d_t dgemm; trsm_d_t dtrsm; syrk_d_t dsyrk; symm_d_t dsymm;
+    potrf_d_t dpotrf;
+    gemm_s_t sgemm; trsm_s_t strsm; syrk_s_t ssyrk; symm_s_t ssymm;
+    potrf_s_t spotrf;
+} mkl;
+
+static int mkl_load(void)
+{
+    static int tried = 0, ok = 0;
+    if (tried) return ok;
+    tried = 1;
+    const char *names[] = {getenv("ORACLE_MKL"),
+                           "/opt/conda/lib/libmkl_rt.so.1",
+                           "/opt/conda/lib/libmkl_rt.so", "libmkl_rt.so.1",
+                           "libmkl_rt.so"};
+    void *h = NULL;
+    for (int i = 0; i < 5 && h == NULL; i++)
+        if (names[i] != NULL) h = dlopen(names[i], RTLD_NOW | RTLD_GLOBAL);
+    if (h == NULL) return 0;
+    /* tile kernels must be sequential: parallelism comes from the tasks */
+    int (*set_layer)(int) = (int (*)(int))dlsym(h, "MKL_Set_Threading_Layer");
+    if (set_layer) set_layer(1 /* MKL_THREADING_SEQUENTIAL */);
+    mkl.dgemm = (gemm_d_t)dlsym(h, "cblas_dgemm");
+    mkl.dtrsm = (trsm_d_t)dlsym(h, "cblas_dtrsm");
+    mkl.dsyrk = (syrk_d_t)dlsym(h, "cblas_dsyrk");
+    mkl.dsymm = (symm_d_t)dlsym(h, "cblas_dsymm");
+    mkl.dpotrf = (potrf_d_t)dlsym(h, "LAPACKE_dpotrf_work");
+    mkl.sgemm = (gemm_s_t)dlsym(h, "cblas_sgemm");
+    mkl.strsm = (trsm_s_t)dlsym(h, "cblas_strsm");
+    mkl.ssyrk = (syrk_s_t)dlsym(h, "cblas_ssyrk");
+    mkl.ssymm = (symm_s_t)dlsym(h, "cblas_ssymm");
+    mkl.spotrf = (potrf_s_t)dlsym(h, "LAPACKE_spotrf_work");
+    ok = mkl.dgemm && mkl.dtrsm && mkl.dsyrk && mkl.dsymm && mkl.dpotrf &&
+         mkl.sgemm && mkl.strsm && mkl.ssyrk && mkl.ssymm && mkl.spotrf;
+    return ok;
+}
+
+int ora_set_blas(int backend)
+{
+    if (backend == ORA_BLAS_MKL && mkl_load()) ora_backend = ORA_BLAS_MKL;
+    else ora_backend = ORA_BLAS_BUILTIN;
+    return ora_backend;
+}
+int ora_get_blas(void) { return ora_backend; }
+
+/* ---------------------------------------------------------------------
+ * Tile descriptor (reference include/plasma_descriptor.h:40-246,
+ * control/descriptor.c:134-264); views are not needed here (i = j = 0).
+ * type: ORA_LOWER / ORA_UPPER triangular, anything else general.
+ * --------------------------------------------------------------------- */
+typedef struct {
+    int type, mb, nb, gm, gn, m, n, mt, nt;
+    size_t A21, A12, A22, es;
+    void *mat;
+} ora_desc_t;
+
+static int ora_desc_create(ora_desc_t *A, int type, size_t es, int nb, int m,
+                           int n)
+{
+    memset(A, 0, sizeof(*A));
+    A->type = (type == ORA_LOWER || type == ORA_UPPER) ? type : 0;
+    A->mb = A->nb = nb;
+    A->gm = A->m = m;
+    A->gn = A->n = n;
+    A->mt = (m + nb - 1) / nb;
+    A->nt = (n + nb - 1) / nb;
+    A->es = es;
+    size_t lm1 = (size_t)(m / nb), ln1 = (size_t)(n / nb), elems;
+    if (A->type) {
+        size_t full = (ln1 * (1 + lm1)) / 2;
+        A->A21 = A->A12 = (size_t)nb * nb * full;
+        A->A22 = (size_t)(m - m % nb) * (n % nb) + A->A12;
+        elems = full * nb * nb + (size_t)m * (n % nb);
+    }
+    else {
+        A->A21 = (size_t)(m - m % nb) * (n - n % nb);
+        A->A12 = (size_t)(m % nb) * (n - n % nb) + A->A21;
+        A->A22 = (size_t)(m - m % nb) * (n % nb) + A->A12;
+        elems = (size_t)m * n;
+    }
+    A->mat = calloc(elems ? elems : 1, es);
+    return A->mat == NULL;
+}
+
+static void ora_desc_destroy(ora_desc_t *A) { free(A->mat); A->mat = NULL; }
+
+static void *ora_tile_addr(const ora_desc_t *A, int m, int n)
+{
+    size_t lm1 = (size_t)(A->gm / A->mb), ln1 = (size_t)(A->gn / A->nb), off;
+    if ((size_t)m < lm1) {
+        if ((size_t)n < ln1) {
+            if (A->type == ORA_UPPER)
+                off = (size_t)A->mb * A->nb * (m + ((size_t)n * (n + 1)) / 2);
+            else if (A->type == ORA_LOWER)
+                off = (size_t)A->mb * A->nb *
+                      ((m - n) + ((size_t)n * (2 * lm1 - n + 1)) / 2);
+            else
+                off = (size_t)A->mb * A->nb * (m + lm1 * n);
+        }
+        else
+            off = A->A12 + (size_t)A->mb * (A->gn % A->nb) * m;
+    }
+    else {
+        if ((size_t)n < ln1) off = A->A21 + (size_t)A->nb * (A->gm % A->mb) * n;
+        else                 off = A->A22;
+    }
+    return (char *)A->mat + off * A->es;
+}
+
+static int ora_mmain(const ora_desc_t *A, int k)
+{
+    return (k < A->gm / A->mb) ? A->mb : A->gm % A->mb;
+}
+static int ora_mview(const ora_desc_t *A, int k)
+{
+    if (k < A->mt - 1) return A->mb;
+    return (A->m % A->mb) ? A->m % A->mb : A->mb;
+}
+static int ora_nview(const ora_desc_t *A, int k)
+{
+    if (k < A->nt - 1) return A->nb;
+    return (A->n % A->nb) ? A->n % A->nb : A->nb;
+}
+
+/* ------------------------- double precision -------------------------- */
+#define REAL double
+#define FN(name) d_##name
+#define CBLAS(name) mkl.d##name
+#define MKL(name) mkl.d##name
+#include "ora_tile.inc"
+#undef REAL
+#undef FN
+#undef CBLAS
+#undef MKL
+
+/* ------------------------- single precision -------------------------- */
+#define REAL float
+#define FN(name) s_##name
+#define CBLAS(name) mkl.s##name
+#define MKL(name) mkl.s##name
+#include "ora_tile.inc"
+#undef REAL
+#undef FN
+#undef CBLAS
+#undef MKL
+
+int ora_dpotrf(int uplo, int n, double *A, int lda, int nb)
+{
+    return d_potrf(uplo, n, A, lda, nb);
+}
+int ora_spotrf(int uplo, int n, float *A, int lda, int nb)
+{
+    return s_potrf(uplo, n, A, lda, nb);
+}
+int ora_dposv(int uplo, int n, int nrhs, double *A, int lda, double *B,
+              int ldb, int nb)
+{
+    return d_solve(uplo, n, nrhs, A, lda, B, ldb, nb, 1);
+}
+int ora_sposv(int uplo, int n, int nrhs, float *A, int lda, float *B,
+              int ldb, int nb)
+{
+    return s_solve(uplo, n, nrhs, A, lda, B, ldb, nb, 1);
+}
+int ora_dpotrs(int uplo, int n, int nrhs, const double *A, int lda,
+               double *B, int ldb, int nb)
+{
+    return d_solve(uplo, n, nrhs, (double *)A, lda, B, ldb, nb, 0);
+}
+
+/* ---------------------------------------------------------------------
+ * Tile layout round trip (reference compute/pztr2desc.c, pzdesc2tr.c,
+ * pzge2desc.c, pzdesc2ge.c).
+ * --------------------------------------------------------------------- */
+long ora_tile_storage_elems(int uplo, int m, int n, int nb)
+{
+    int type = (uplo == ORA_LOWER || uplo == ORA_UPPER) ? uplo : 0;
+    size_t lm1 = (size_t)(m / nb), ln1 = (size_t)(n / nb);
+    if (type) return (long)(((ln1 * (1 + lm1)) / 2) * nb * nb + (size_t)m * (n % nb));
+    return (long)m * n;
+}
+
+void ora_lapack2tile(int uplo, int m, int n, const double *pA, int lda,
+                     double *tiles, int nb)
+{
+    ora_desc_t A;
+    if (ora_desc_create(&A, uplo, sizeof(double), nb, m, n)) return;
+    d_lapack2tile(uplo, pA, lda, A, 1);
+    memcpy(tiles, A.mat, (size_t)ora_tile_storage_elems(uplo, m, n, nb) * sizeof(double));
+    ora_desc_destroy(&A);
+}
+
+void ora_tile2lapack(int uplo, int m, int n, const double *tiles, double *pA,
+                     int lda, int nb)
+{
+    ora_desc_t A;
+    if (ora_desc_create(&A, uplo, sizeof(double), nb, m, n)) return;
+    memcpy(A.mat, tiles, (size_t)ora_tile_storage_elems(uplo, m, n, nb) * sizeof(double));
+    d_lapack2tile(uplo, pA, lda, A, 0);
+    ora_desc_destroy(&A);
+}
+
+/* ---------------------------------------------------------------------
+ * Precision conversion (reference core_zlag2c.c:48-79 -> LAPACK dlag2s:
+ * info = 1 when an entry exceeds the FP32 range; core_clag2z.c).
+ * --------------------------------------------------------------------- */
+int ora_dlag2s(int m, int n, const double *A, int lda, float *As, int ldas)
+{
+    double rmax = (double)FLT_MAX;
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < m; i++) {
+            double v = A[(size_t)j * lda + i];
+            if (v < -rmax || v > rmax) return 1;
+            As[(size_t)j * ldas + i] = (float)v;
+        }
+    return 0;
+}
+
+void ora_slag2d(int m, int n, const float *As, int ldas, double *A, int lda)
+{
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < m; i++)
+            A[(size_t)j * lda + i] = (double)As[(size_t)j * ldas + i];
+}
+
+/* ---------------------------------------------------------------------
+ * Mixed-precision iterative refinement (reference compute/zcposv.c:363-539
+ * generated as dsposv).  Tile storage: A, B, X, R general FP64; As, Xs
+ * general FP32.  Steps and constants as in the reference: itermax = 30,
+ * eps = dlamch('E') = 2^-53, cte = ||A||_inf eps sqrt(n), convergence when
+ * every column has max|r| <= max|x| * cte.  A failure of the FP32 factor or
+ * an FP64 -> FP32 overflow fails the whole sequence (no FP64 fallback), as
+ * in the reference.
+ * --------------------------------------------------------------------- */
+
+/* ||A||_inf of the symmetric matrix held in the uplo triangle
+ * (reference compute/pzlanhe.c:83-131). */
+static double d_lansy_inf(int uplo, int n, const double *A, int lda)
+{
+    double best = 0;
+    for (int i = 0; i < n; i++) {
+        double s = 0;
+        for (int j = 0; j < n; j++) {
+            int r = i, c = j;
+            if ((uplo == ORA_LOWER && r < c) || (uplo == ORA_UPPER && r > c)) {
+                r = j;
+                c = i;
+            }
+            s += fabs(A[(size_t)c * lda + r]);
+        }
+        if (s > best || isnan(s)) best = s;
+    }
+    return best;
+}
+
+/* column max |x| (reference compute/pdzamax.c:36-52, core_dzamax.c) */
+static void d_amax_cols(int m, int n, const double *X, int ldx, double *v)
+{
+    for (int j = 0; j < n; j++) {
+        double t = fabs(X[(size_t)j * ldx]);
+        for (int i = 1; i < m; i++) {
+            double a = fabs(X[(size_t)j * ldx + i]);
+            if (a > t) t = a;
+        }
+        v[j] = t;
+    }
+}
+
+/* reference zcposv.c:275-287 */
+static int conv(const double *rn, const double *xn, int n, double cte)
+{
+    for (int i = 0; i < n; i++)
+        if (rn[i] > xn[i] * cte) return 0;
+    return 1;
+}
+
+/* Solve As Xs = Rs in FP32 on tiles: two sweeps with the FP32 factor. */
+static int s_sweeps(int uplo, ora_desc_t As, ora_desc_t Xs)
+{
+    volatile int status = 0;
+    #pragma omp parallel num_threads(ora_nthreads())
+    #pragma omp master
+    {
+        s_pxtrsm_left(uplo, uplo == ORA_UPPER ? ORA_TRANS : ORA_NOTRANS, As, Xs,
+                      &status);
+        s_pxtrsm_left(uplo, uplo == ORA_UPPER ? ORA_NOTRANS : ORA_TRANS, As, Xs,
+                      &status);
+    }
+    return status;
+}
+
+int ora_dsposv(int uplo, int n, int nrhs, const double *pA, int lda,
+               const double *pB, int ldb, double *pX, int ldx, int *iter,
+               int nb)
+{
+    const int itermax = 30;
+    *iter = 0;
+    if (n == 0 || nrhs == 0) return 0;
+
+    /* FP64 pieces (norms, residual, update) run on LAPACK-layout copies;
+     * the FP32 factor and sweeps run on tiles.  Results equal the
+     * reference's tile computation up to the summation order of the
+     * residual product. */
+    double *A = malloc((size_t)n * n * sizeof(double));
+    double *B = malloc((size_t)n * nrhs * sizeof(double));
+    double *X = malloc((size_t)n * nrhs * sizeof(double));
+    double *R = malloc((size_t)n * nrhs * sizeof(double));
+    float *As = malloc((size_t)n * n * sizeof(float));
+    float *Xs = malloc((size_t)n * nrhs * sizeof(float));
+    double *xn = malloc(nrhs * sizeof(double)), *rn = malloc(nrhs * sizeof(double));
+    int info = 0;
+    for (int j = 0; j < n; j++)
+        memcpy(A + (size_t)j * n, pA + (size_t)j * lda, n * sizeof(double));
+    for (int j = 0; j < nrhs; j++)
+        memcpy(B + (size_t)j * n, pB + (size_t)j * ldb, n * sizeof(double));
+
+    const double eps = ldexp(1.0, -53);
+    const double anorm = d_lansy_inf(uplo, n, A, n);
+    ora_desc_t tAs, tXs;
+    ora_desc_create(&tAs, 0, sizeof(float), nb, n, n);
+    ora_desc_create(&tXs, 0, sizeof(float), nb, n, nrhs);
+
+    /* Xs = lag2s(B), As = lag2s(A) (reference :445-449) */
+    if (ora_dlag2s(n, nrhs, B, n, Xs, n) || ora_dlag2s(n, n, A, n, As, n)) {
+        info = 1;
+        goto done;
+    }
+    /* FP32 tile factorization and solve (reference :452-460) */
+    {
+        volatile int status = 0;
+        s_lapack2tile(0, As, n, tAs, 1);
+        #pragma omp parallel num_threads(ora_nthreads())
+        #pragma omp master
+        {
+            s_pxpotrf(uplo, tAs, &status);
+        }
+        if (status) { info = status; goto done; }
+    }
+    s_lapack2tile(0, Xs, n, tXs, 1);
+    if ((info = s_sweeps(uplo, tAs, tXs))) goto done;
+    s_lapack2tile(0, Xs, n, tXs, 0);
+    ora_slag2d(n, nrhs, Xs, n, X, n);
+
+    /* R = B - A X; norms; convergence (reference :466-481) */
+    memcpy(R, B, (size_t)n * nrhs * sizeof(double));
+    d_k_symm(uplo, n, nrhs, -1.0, A, n, X, n, 1.0, R, n);
+    d_amax_cols(n, nrhs, X, n, xn);
+    d_amax_cols(n, nrhs, R, n, rn);
+    const double cte = anorm * eps * sqrt((double)n);
+    if (conv(rn, xn, nrhs, cte)) {
+        *iter = 0;
+        goto done;
+    }
+    for (int it = 0; it < itermax; it++) {
+        /* reference :487-516 */
+        if (ora_dlag2s(n, nrhs, R, n, Xs, n)) { info = 1; goto done; }
+        s_lapack2tile(0, Xs, n, tXs, 1);
+        if ((info = s_sweeps(uplo, tAs, tXs))) goto done;
+        s_lapack2tile(0, Xs, n, tXs, 0);
+        ora_slag2d(n, nrhs, Xs, n, R, n);
+        for (size_t e = 0; e < (size_t)n * nrhs; e++) X[e] += R[e];
+        memcpy(R, B, (size_t)n * nrhs * sizeof(double));
+        d_k_symm(uplo, n, nrhs, -1.0, A, n, X, n, 1.0, R, n);
+        d_amax_cols(n, nrhs, X, n, xn);
+        d_amax_cols(n, nrhs, R, n, rn);
+        if (conv(rn, xn, nrhs, cte)) {
+            *iter = it + 1;
+            goto done;
+        }
+    }
+    /* FP64 fallback (reference :524-538) */
+    *iter = -itermax - 1;
+    memcpy(X, B, (size_t)n * nrhs * sizeof(double));
+    info = d_solve(uplo, n, nrhs, A, n, X, n, nb, 1);
+
+done:
+    if (info == 0)
+        for (int j = 0; j < nrhs; j++)
+            memcpy(pX + (size_t)j * ldx, X + (size_t)j * n, n * sizeof(double));
+    ora_desc_destroy(&tAs);
+    ora_desc_destroy(&tXs);
+    free(A); free(B); free(X); free(R); free(As); free(Xs); free(xn); free(rn);
+    return info;
+}

@@ -1,0 +1,76 @@
+/*
+ * plasma_oracle.h -- CPU restatement of PLASMA's tile Cholesky solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the correctness oracle and the
+ * CPU baseline of the MI355X implementation; nothing in the product links
+ * or calls it.
+ *
+ * It restates the reference's algorithms on the reference's own data
+ * layout: LAPACK arrays are translated into PLASMA tile storage (the
+ * triangular / general descriptors of include/plasma_descriptor.h), the
+ * tile algorithms insert one OpenMP task per tile operation with the same
+ * depend clauses as the reference (compute/pzpotrf.c, pztrsm.c, zcposv.c
+ * ...), and the results are translated back.  Tile kernels are either this
+ * file's own loops (ORA_BLAS_BUILTIN) or the vendor BLAS/LAPACK the
+ * reference calls (ORA_BLAS_MKL, MKL loaded at run time with dlopen).
+ *
+ * Parity pins (see tests/test_oracle.py): every routine is checked against
+ * an independent LAPACK (scipy / OpenBLAS) on the reference tester's
+ * criteria, against closed forms and against brute force on small inputs.
+ */
+#ifndef PLASMA_ORACLE_H
+#define PLASMA_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORA_BLAS_BUILTIN = 0, ORA_BLAS_MKL = 1 };
+enum { ORA_UPPER = 121, ORA_LOWER = 122 };
+
+/* Select the tile-kernel backend; returns the backend in use (MKL falls
+ * back to built-in when it cannot be loaded). */
+int ora_set_blas(int backend);
+int ora_get_blas(void);
+/* Threads used by the OpenMP task runtime (0 = keep the default). */
+void ora_set_threads(int nthreads);
+int ora_get_threads(void);
+
+/* Test hook: plant a known mistake in the tile Cholesky (1 = drop the
+ * syrk update of diagonal tiles).  0 restores the correct algorithm. */
+void ora_plant_mistake(int which);
+
+/* plasma_dpotrf / dpotrs / dposv with tile size nb (reference
+ * compute/zpotrf.c:74-150, zpotrs.c:77-172, zposv.c:90-186). */
+int ora_dpotrf(int uplo, int n, double *A, int lda, int nb);
+int ora_dpotrs(int uplo, int n, int nrhs, const double *A, int lda,
+               double *B, int ldb, int nb);
+int ora_dposv(int uplo, int n, int nrhs, double *A, int lda, double *B,
+              int ldb, int nb);
+int ora_spotrf(int uplo, int n, float *A, int lda, int nb);
+int ora_sposv(int uplo, int n, int nrhs, float *A, int lda, float *B,
+              int ldb, int nb);
+
+/* plasma_dsposv (reference compute/zcposv.c:116-271, :363-539). */
+int ora_dsposv(int uplo, int n, int nrhs, const double *A, int lda,
+               const double *B, int ldb, double *X, int ldx, int *iter,
+               int nb);
+
+/* plasma_dlag2s / plasma_slag2d on LAPACK arrays. */
+int ora_dlag2s(int m, int n, const double *A, int lda, float *As, int ldas);
+void ora_slag2d(int m, int n, const float *As, int ldas, double *A, int lda);
+
+/* Tile layout round trip: copy the uplo (or general when uplo is neither)
+ * tiles of A into PLASMA tile storage and write the storage image to
+ * `tiles` (size from ora_tile_storage_elems), then translate back into B. */
+long ora_tile_storage_elems(int uplo, int m, int n, int nb);
+void ora_lapack2tile(int uplo, int m, int n, const double *A, int lda,
+                     double *tiles, int nb);
+void ora_tile2lapack(int uplo, int m, int n, const double *tiles, double *A,
+                     int lda, int nb);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

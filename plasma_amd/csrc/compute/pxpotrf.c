@@ -1,0 +1,219 @@
+/*
+ * pxpotrf.c -- tile Cholesky as a lookahead-scheduled, stream-ordered
+ * launch sequence.
+ *
+ * The reference inserts one OpenMP task per tile operation and lets the
+ * runtime order them by their depend clauses (compute/pzpotrf.c:37-80 for
+ * lower, :84-126 for upper; 341 376 gemm tasks at n = 32768, nb = 256).
+ * Here every step k issues three launches that each cover all tiles of one
+ * task class:
+ *     potrf(k)            diagonal tile                 (core_zpotrf)
+ *     trsm(k)             panel tiles m > k             (core_ztrsm)
+ *     update(k, n0, n1)   trailing tiles, columns n0..n1 (core_zherk/zgemm)
+ * With lookahead depth >= 1 the update of step k is split: column k+1 is
+ * updated first on the main stream, the panel stream then factors column
+ * k+1 (potrf + trsm) while the main stream updates the remaining columns.
+ * Events carry the two dependencies the depend clauses expressed:
+ *     panel(k+1) after update(k, k+1)           (ev_col)
+ *     update(k+1, ...) after panel(k+1)         (ev_panel)
+ * so the critical path potrf -> trsm -> next column overlaps the bulk
+ * update.  The streams are joined again before returning.
+ */
+#include "internal.h"
+
+static int issue(int err, plasma_sequence_t *sequence,
+                 plasma_request_t *request)
+{
+    if (err != 0)
+        plasma_request_fail(sequence, request, err);
+    return err;
+}
+
+/* Algorithmic flops of the launches (reference test/flops.h conventions:
+ * potrf n^3/3 + n^2/2 + n/6, trsm m n^2, syrk k n (n + 1), gemm 2 m n k). */
+static double potrf_flops(double n)
+{
+    return n * n * n / 3.0 + n * n / 2.0 + n / 6.0;
+}
+
+static double panel_flops(plasma_desc_t A, int k)
+{
+    double f = 0, nb = A.nb;
+    for (int m = k + 1; m < A.mt; m++)
+        f += plasma_tile_mview(A, m) * nb * nb;
+    return f;
+}
+
+static double update_flops(plasma_desc_t A, int n0, int n1)
+{
+    double f = 0, nb = A.nb;
+    for (int n = n0; n < n1 && n < A.mt; n++) {
+        double nv = plasma_tile_mview(A, n);
+        f += nb * nv * (nv + 1);                       /* syrk on (n, n) */
+        for (int m = n + 1; m < A.mt; m++)
+            f += 2.0 * plasma_tile_mview(A, m) * nv * nb;  /* gemm */
+    }
+    return f;
+}
+
+static int k_potrf(plasma_enum_t uplo, plasma_desc_t A, int k, int *st,
+                   hipStream_t s)
+{
+    int slot;
+    plasma_prof_begin(s, PlasmaProfPotrf, potrf_flops(plasma_tile_mview(A, k)),
+                      0, &slot);
+    int err = plasma_kern_potrf_diag(uplo, A, k, st, s);
+    plasma_prof_end(s, slot);
+    return err;
+}
+
+static int k_panel(plasma_enum_t uplo, plasma_desc_t A, int k, int *st,
+                   hipStream_t s)
+{
+    int slot;
+    plasma_prof_begin(s, PlasmaProfPanel, panel_flops(A, k), 0, &slot);
+    int err = plasma_kern_trsm_panel(uplo, A, k, st, s);
+    plasma_prof_end(s, slot);
+    return err;
+}
+
+static int k_update(plasma_enum_t uplo, plasma_desc_t A, int k, int n0,
+                    int n1, int *st, hipStream_t s)
+{
+    int slot;
+    plasma_context_t *ctx = plasma_context_self();
+    plasma_prof_begin(s, PlasmaProfUpdate,
+                      ctx->profile ? update_flops(A, n0, n1) : 0, 0, &slot);
+    int err = plasma_kern_trailing_update(uplo, A, k, n0, n1, st, s);
+    plasma_prof_end(s, slot);
+    return err;
+}
+
+void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A,
+                    plasma_sequence_t *sequence, plasma_request_t *request)
+{
+    if (sequence->status != PlasmaSuccess)
+        return;
+    plasma_context_t *ctx = plasma_context_self();
+    int *st = plasma_sequence_dstatus(sequence, request);
+    hipStream_t s0 = ctx->stream, s1 = ctx->panel_stream;
+    const int mt = A.mt;
+
+    if (ctx->lookahead == 0 || mt < 3) {
+        for (int k = 0; k < mt; k++) {
+            if (issue(k_potrf(uplo, A, k, st, s0),
+                      sequence, request) ||
+                issue(k_panel(uplo, A, k, st, s0),
+                      sequence, request) ||
+                issue(k_update(uplo, A, k, k + 1, mt, st, s0), sequence, request))
+                return;
+        }
+        return;
+    }
+
+    hipEvent_t ev_fork, ev_col, ev_panel;
+    if (plasma_hip_check(hipEventCreateWithFlags(&ev_fork,
+                                                 hipEventDisableTiming),
+                         "hipEventCreate") ||
+        plasma_hip_check(hipEventCreateWithFlags(&ev_col,
+                                                 hipEventDisableTiming),
+                         "hipEventCreate") ||
+        plasma_hip_check(hipEventCreateWithFlags(&ev_panel,
+                                                 hipEventDisableTiming),
+                         "hipEventCreate")) {
+        plasma_request_fail(sequence, request, PlasmaErrorInternal);
+        return;
+    }
+
+    /* fork: the panel stream starts after everything queued so far */
+    hipEventRecord(ev_fork, s0);
+    hipStreamWaitEvent(s1, ev_fork, 0);
+
+    int err = 0;
+    err = issue(k_potrf(uplo, A, 0, st, s1), sequence, request);
+    if (!err)
+        err = issue(k_panel(uplo, A, 0, st, s1), sequence,
+                    request);
+    hipEventRecord(ev_panel, s1);
+
+    for (int k = 0; k < mt && !err; k++) {
+        /* main stream: wait for panel k, update column k+1 */
+        hipStreamWaitEvent(s0, ev_panel, 0);
+        if (k + 1 < mt) {
+            err = issue(k_update(uplo, A, k, k + 1, k + 2, st, s0),
+                        sequence, request);
+            hipEventRecord(ev_col, s0);
+            /* panel stream: factor column k+1 */
+            hipStreamWaitEvent(s1, ev_col, 0);
+            if (!err)
+                err = issue(k_potrf(uplo, A, k + 1, st, s1),
+                            sequence, request);
+            if (!err)
+                err = issue(k_panel(uplo, A, k + 1, st, s1),
+                            sequence, request);
+            hipEventRecord(ev_panel, s1);
+            /* main stream: the rest of the trailing matrix */
+            if (!err && k + 2 < mt)
+                err = issue(k_update(uplo, A, k, k + 2, mt, st, s0),
+                            sequence, request);
+        }
+    }
+    /* join */
+    hipEventRecord(ev_panel, s1);
+    hipStreamWaitEvent(s0, ev_panel, 0);
+    hipEventDestroy(ev_fork);
+    hipEventDestroy(ev_col);
+    hipEventDestroy(ev_panel);
+}
+
+/*
+ * Triangular sweeps (reference compute/pztrsm.c, Left side).  In the
+ * lower-equivalent convention the four (uplo, trans) cases used by potrs
+ * reduce to a forward sweep with L or a backward sweep with L^T.
+ */
+void plasma_pxtrsm_left(plasma_enum_t uplo, plasma_enum_t trans,
+                        plasma_desc_t A, plasma_desc_t B,
+                        plasma_sequence_t *sequence,
+                        plasma_request_t *request)
+{
+    if (sequence->status != PlasmaSuccess)
+        return;
+    plasma_context_t *ctx = plasma_context_self();
+    int *st = plasma_sequence_dstatus(sequence, request);
+    hipStream_t s0 = ctx->stream;
+    /* lower+NoTrans and upper+Trans solve with L; the others with L^T */
+    const int backward = (uplo == PlasmaLower) ? (trans != PlasmaNoTrans)
+                                               : (trans == PlasmaNoTrans);
+    const int mt = B.mt;
+    for (int i = 0; i < mt; i++) {
+        const int k = backward ? mt - 1 - i : i;
+        int slot;
+        double nbk = plasma_tile_mview(B, k);
+        int nupd = backward ? k : mt - 1 - k;
+        plasma_prof_begin(s0, PlasmaProfSweep,
+                          nbk * nbk * B.n + 2.0 * nbk * nupd * B.mb * B.n, 0,
+                          &slot);
+        int err = issue(plasma_kern_trsm_diag_left(uplo, backward, A, B, k,
+                                                   st, s0),
+                        sequence, request);
+        if (!err)
+            err = issue(plasma_kern_sweep_update(uplo, backward, A, B, k, st,
+                                                 s0),
+                        sequence, request);
+        plasma_prof_end(s0, slot);
+        if (err)
+            return;
+    }
+}
+
+void plasma_pxpotrs(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
+                    plasma_sequence_t *sequence, plasma_request_t *request)
+{
+    /* reference compute/zpotrs.c:270-283 */
+    plasma_pxtrsm_left(uplo, uplo == PlasmaUpper ? PlasmaConjTrans
+                                                 : PlasmaNoTrans,
+                       A, B, sequence, request);
+    plasma_pxtrsm_left(uplo, uplo == PlasmaUpper ? PlasmaNoTrans
+                                                 : PlasmaConjTrans,
+                       A, B, sequence, request);
+}

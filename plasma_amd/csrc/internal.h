@@ -1,0 +1,154 @@
+/*
+ * internal.h -- private state and helpers of the MI355X PLASMA library.
+ *
+ * The host side is C.  It owns the context (device, streams, status words,
+ * device workspace pool, graph cache) and schedules the tile algorithms as
+ * stream-ordered kernel launches through the thin C ABI declared in
+ * kernels/plasma_kernels.h.
+ */
+#ifndef PLASMA_INTERNAL_H
+#define PLASMA_INTERNAL_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#include "plasma.h"
+#include "kernels/plasma_kernels.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+static inline int imin(int a, int b) { return a < b ? a : b; }
+static inline int imax(int a, int b) { return a > b ? a : b; }
+
+/* Device status words: one per live sequence (see control/async.c). */
+#define PLASMA_STATUS_SLOTS 256
+
+/* Cached device allocations kept by the workspace pool. */
+#define PLASMA_POOL_ENTRIES 64
+
+typedef struct {
+    void  *ptr;
+    size_t bytes;
+    int    in_use;
+} plasma_pool_entry_t;
+
+/* profiler records (control/profile.c) */
+#define PLASMA_PROF_MAX 16384
+typedef struct {
+    hipEvent_t beg, end;
+    double flops, bytes;
+    int cls;
+} plasma_prof_rec_t;
+
+struct plasma_context_s {
+    /* reference parameters (control/context.c:165-180) */
+    int tuning;
+    int nb;
+    int ib;
+    int inplace_outplace;
+    int max_threads;
+    int max_panel_threads;
+    int householder_mode;
+
+    /* extensions */
+    int lookahead;        /* panel lookahead depth, 0 disables */
+    int graph_capture;    /* replay cached hipGraphs for the sync drivers */
+
+    /* device */
+    int device;
+    int num_cus;
+    hipStream_t stream;        /* main stream: trailing updates, sweeps */
+    hipStream_t panel_stream;  /* lookahead stream: diagonal potrf + panel */
+    int owns_streams;
+
+    /* status words, one per sequence */
+    int *d_status;                                   /* device array */
+    const plasma_sequence_t *slot_seq[PLASMA_STATUS_SLOTS];
+    plasma_request_t *slot_req[PLASMA_STATUS_SLOTS];
+    int slot_next;
+
+    /* device scratch for reductions (norms, convergence flags) */
+    double *d_scalars;          /* PLASMA_SCALARS doubles */
+
+    /* workspace pool */
+    plasma_pool_entry_t pool[PLASMA_POOL_ENTRIES];
+
+    /* kernel profiler */
+    int profile;
+    int nprof;
+    plasma_prof_rec_t *prof;
+    double prof_ms[PlasmaProfClasses];
+    double prof_flops[PlasmaProfClasses];
+    double prof_bytes[PlasmaProfClasses];
+    long prof_count[PlasmaProfClasses];
+};
+
+#define PLASMA_SCALARS 64
+
+
+void plasma_prof_begin(hipStream_t s, int cls, double flops, double bytes,
+                       int *slot);
+void plasma_prof_end(hipStream_t s, int slot);
+
+/* context.c */
+plasma_context_t *plasma_context_self(void);
+int plasma_hip_check(hipError_t err, const char *what);
+
+/* async.c: the device status word of a sequence (allocated on first use) */
+int *plasma_sequence_dstatus(plasma_sequence_t *sequence,
+                             plasma_request_t *request);
+
+/* workspace.c: device memory from the pool */
+void *plasma_dev_alloc(size_t bytes);
+void  plasma_dev_free(void *ptr);
+void  plasma_pool_release_all(plasma_context_t *ctx);
+
+/* Pointer kind of a caller's LAPACK-layout array. */
+int plasma_is_device_ptr(const void *p);
+
+/*
+ * Parallel tile algorithms (compute/p*.c).  Each enqueues its launches on
+ * the context streams and returns; the main stream is the join point.
+ */
+void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A,
+                    plasma_sequence_t *sequence, plasma_request_t *request);
+void plasma_pxpotrs(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
+                    plasma_sequence_t *sequence, plasma_request_t *request);
+void plasma_pxtrsm_left(plasma_enum_t uplo, plasma_enum_t trans,
+                        plasma_desc_t A, plasma_desc_t B,
+                        plasma_sequence_t *sequence,
+                        plasma_request_t *request);
+
+/* layout translation between a LAPACK array (host or device) and tiles */
+void plasma_pxlapack2desc(plasma_enum_t region, const void *pA, int lda,
+                          plasma_desc_t A, plasma_sequence_t *sequence,
+                          plasma_request_t *request);
+void plasma_pxdesc2lapack(plasma_enum_t region, plasma_desc_t A, void *pA,
+                          int lda, plasma_sequence_t *sequence,
+                          plasma_request_t *request);
+
+/* precision-generic drivers behind plasma_{d,s}{potrf,potrs,posv} */
+int plasma_xpotrf(plasma_enum_t precision, plasma_enum_t uplo, int n,
+                  void *pA, int lda);
+int plasma_xpotrs(plasma_enum_t precision, plasma_enum_t uplo, int n,
+                  int nrhs, void *pA, int lda, void *pB, int ldb);
+int plasma_xposv(plasma_enum_t precision, plasma_enum_t uplo, int n,
+                 int nrhs, void *pA, int lda, void *pB, int ldb);
+void plasma_omp_xpotrf(plasma_enum_t precision, plasma_enum_t uplo,
+                       plasma_desc_t A, plasma_sequence_t *sequence,
+                       plasma_request_t *request);
+void plasma_omp_xpotrs(plasma_enum_t precision, plasma_enum_t uplo,
+                       plasma_desc_t A, plasma_desc_t B,
+                       plasma_sequence_t *sequence, plasma_request_t *request);
+void plasma_omp_xposv(plasma_enum_t precision, plasma_enum_t uplo,
+                      plasma_desc_t A, plasma_desc_t B,
+                      plasma_sequence_t *sequence, plasma_request_t *request);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PLASMA_INTERNAL_H */

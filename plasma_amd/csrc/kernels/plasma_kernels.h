@@ -1,0 +1,111 @@
+/*
+ * plasma_kernels.h -- the thin C ABI between the C host code and the HIP
+ * kernels for gfx950.
+ *
+ * Every launcher enqueues on the given stream and returns 0, or a negative
+ * PlasmaError code when the launch itself fails.  Launchers take
+ * descriptors by value (their `matrix` is device memory) and the device
+ * status word of the sequence: every kernel returns immediately when that
+ * word is non-zero, which is how the tile tasks of the reference skip their
+ * work after a failure (reference core_blas/core_zpotrf.c:74).
+ *
+ * "Lower-equivalent" convention: for uplo == PlasmaUpper the kernels work on
+ * L = U^T, reading the stored upper tiles transposed, so one algorithm
+ * serves both triangles.
+ */
+#ifndef PLASMA_KERNELS_H
+#define PLASMA_KERNELS_H
+
+#include <hip/hip_runtime_api.h>
+#include "plasma_descriptor.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Fill n ints with v (a kernel, never hipMemset: safe under graph capture). */
+int plasma_kern_fill_int(int *p, int v, int n, hipStream_t s);
+
+/*
+ * LAPACK column-major <-> tile layout (reference compute/pztr2desc.c,
+ * pzdesc2tr.c, pzge2desc.c, pzdesc2ge.c).  region: PlasmaGeneral copies all
+ * tiles of the view, PlasmaLower the tiles m >= n, PlasmaUpper m <= n.
+ * pA is the device address of the view's first element, leading dim lda.
+ */
+int plasma_kern_lapack2tile(plasma_enum_t region, const void *pA, int lda,
+                            plasma_desc_t A, const int *status,
+                            hipStream_t s);
+int plasma_kern_tile2lapack(plasma_enum_t region, plasma_desc_t A, void *pA,
+                            int lda, const int *status, hipStream_t s);
+
+/*
+ * Cholesky factorization of diagonal tile k (reference core_zpotrf.c:54-82
+ * called from pzpotrf.c:41-45).  On a non positive (or NaN) pivot at local
+ * column j the status word is set to nb*k + j + 1 if it was 0.
+ */
+int plasma_kern_potrf_diag(plasma_enum_t uplo, plasma_desc_t A, int k,
+                           int *status, hipStream_t s);
+
+/* Panel solve A(m,k) <- A(m,k) L(k,k)^{-T} for m > k
+ * (reference pzpotrf.c:47-57; upper: :90-99). */
+int plasma_kern_trsm_panel(plasma_enum_t uplo, plasma_desc_t A, int k,
+                           const int *status, hipStream_t s);
+
+/* Trailing update A(m,n) -= A(m,k) A(n,k)^T for n in [n_first, n_end) and
+ * m in [n, mt), lower triangle only on diagonal tiles
+ * (reference pzpotrf.c:58-78, syrk + gemm tasks; upper: :100-125). */
+int plasma_kern_trailing_update(plasma_enum_t uplo, plasma_desc_t A, int k,
+                                int n_first, int n_end, const int *status,
+                                hipStream_t s);
+
+/*
+ * Triangular sweeps of potrs (reference pztrsm.c:113-177 and :43-107).
+ * backward == 0: solve L Y = B (forward); backward == 1: L^T X = Y.
+ * trsm_diag solves with diagonal tile k on every tile column of B(k,:);
+ * sweep_update subtracts the contribution of block row k from the block rows
+ * still to be solved (m > k forward, m < k backward).
+ */
+int plasma_kern_trsm_diag_left(plasma_enum_t uplo, int backward,
+                               plasma_desc_t A, plasma_desc_t B, int k,
+                               const int *status, hipStream_t s);
+int plasma_kern_sweep_update(plasma_enum_t uplo, int backward,
+                             plasma_desc_t A, plasma_desc_t B, int k,
+                             const int *status, hipStream_t s);
+
+/*
+ * Mixed-precision helpers (reference compute/zcposv.c:363-539).
+ */
+/* FP64 -> FP32 (round to nearest), sets *status = 1 on overflow
+ * (reference core_zlag2c.c:48-79); or FP32 -> FP64 (core_clag2z.c). */
+int plasma_kern_lag2(plasma_desc_t src, plasma_desc_t dst, int *status,
+                     hipStream_t s);
+/* B <- A (general, reference pzlacpy.c:118-140) */
+int plasma_kern_lacpy(plasma_desc_t A, plasma_desc_t B, const int *status,
+                      hipStream_t s);
+/* B <- alpha A + beta B (reference pzgeadd.c:41-49) */
+int plasma_kern_geadd(double alpha, plasma_desc_t A, double beta,
+                      plasma_desc_t B, const int *status, hipStream_t s);
+/* ||A||_inf of a symmetric matrix stored in one triangle of a general
+ * descriptor (reference pzlanhe.c:83-131); work: A.n doubles;
+ * result -> *value (device). */
+int plasma_kern_lansy_inf(plasma_enum_t uplo, plasma_desc_t A, double *work,
+                          double *value, const int *status, hipStream_t s);
+/* values[j] = max_i |A(i,j)| (reference pdzamax.c:36-52) */
+int plasma_kern_amax_cols(plasma_desc_t A, double *values,
+                          const int *status, hipStream_t s);
+/* R <- R - A X, A symmetric stored in the uplo triangle of a general
+ * descriptor (reference pzhemm.c:44-80 with alpha = -1, beta = 1). */
+int plasma_kern_symm_residual(plasma_enum_t uplo, plasma_desc_t A,
+                              plasma_desc_t X, plasma_desc_t R,
+                              const int *status, hipStream_t s);
+/* *flag = all_j (rnorm[j] <= xnorm[j] * cte), cte = anorm * eps * sqrt(n)
+ * (reference zcposv.c:275-287 and :476) */
+int plasma_kern_conv_test(const double *rnorm, const double *xnorm, int nrhs,
+                          const double *anorm, double eps_sqrtn, int *flag,
+                          const int *status, hipStream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PLASMA_KERNELS_H */

@@ -1,0 +1,174 @@
+"""GPU parity tests of the FP64/FP32 Cholesky path through the C ABI.
+
+The checks are the reference tester's (test/test_zpotrf.c:124-152,
+test_zposv.c:144-168, test_zpotrs.c:308-331) against the CPU oracle
+(a restatement of PLASMA's tile algorithm, itself pinned to LAPACK in
+test_oracle.py):
+  * potrf: ||L - L_oracle||_F / ||L_oracle||_F <= 1e-12, and the entries the
+    factorization must not touch are bit-identical to the input;
+  * posv / potrs: scaled residual ||B - AX||_inf / (n ||A||_inf ||X||_inf)
+    < 10 * eps64 (stricter than the tester's 50 eps);
+  * a non positive definite input returns LAPACK's info and leaves pA as
+    it was (the reference skips its desc2tr copy on a failed sequence).
+"""
+import numpy as np
+import pytest
+
+import oracle as ora
+import plasma_amd as pa
+from tests import _util as U
+
+pytestmark = pytest.mark.gpu
+
+L, Up = pa.PlasmaLower, pa.PlasmaUpper
+
+CASES = [(1, 16), (15, 16), (100, 16), (256, 256), (257, 64), (300, 128),
+         (1000, 256), (1536, 512), (2048, 256)]
+
+
+def _oracle_factor(uplo, a, nb):
+    ora.set_blas(ora.MKL)
+    f, info = ora.potrf(uplo, a, nb)
+    return f, info
+
+
+@pytest.mark.parametrize("uplo", [L, Up])
+@pytest.mark.parametrize("n,nb", CASES)
+def test_dpotrf_host(plasma, nb_guard, uplo, n, nb):
+    pa.set_param(pa.PlasmaTuning, pa.PlasmaDisabled)
+    pa.set_param(pa.PlasmaNb, nb)
+    lda = n + 3
+    A = U.tester_spd(n, lda)
+    a = A.copy(order="F")
+    info = pa.potrf(uplo, n, a, lda)
+    assert info == 0
+    ref, rinfo = _oracle_factor(uplo, A[:n, :n], nb)
+    assert rinfo == 0
+    err = U.factor_error(a[:n, :n], ref, uplo == L)
+    assert err <= 1e-12, err
+    # padding rows untouched
+    assert np.array_equal(a[n:, :], A[n:, :])
+
+
+@pytest.mark.parametrize("uplo", [L, Up])
+@pytest.mark.parametrize("n,nb", [(100, 16), (1000, 256), (2048, 256)])
+def test_dpotrf_device(plasma, nb_guard, uplo, n, nb):
+    import torch
+    pa.set_param(pa.PlasmaTuning, pa.PlasmaDisabled)
+    pa.set_param(pa.PlasmaNb, nb)
+    A = U.ggt_spd(n, seed=n)
+    # column-major on the device: tensor row j holds column j
+    d = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()
+    torch.cuda.synchronize()
+    info = pa.potrf(uplo, n, d, n)
+    assert info == 0
+    got = d.cpu().numpy().T
+    ref, _ = _oracle_factor(uplo, A, nb)
+    assert U.factor_error(got, ref, uplo == L) <= 1e-12
+
+
+@pytest.mark.parametrize("uplo", [L, Up])
+@pytest.mark.parametrize("n,nb,nrhs", [(15, 16, 3), (300, 128, 17),
+                                       (1000, 256, 64), (2048, 256, 64),
+                                       (1536, 512, 100)])
+def test_dposv(plasma, nb_guard, uplo, n, nb, nrhs):
+    pa.set_param(pa.PlasmaTuning, pa.PlasmaDisabled)
+    pa.set_param(pa.PlasmaNb, nb)
+    A = U.ggt_spd(n, seed=2 * n + 1)
+    B = U.rhs(n, nrhs, seed=n)
+    a, b = A.copy(order="F"), B.copy(order="F")
+    assert pa.posv(uplo, n, nrhs, a, n, b, n) == 0
+    res = U.scaled_residual(A, b, B)
+    assert res < 10 * U.EPS64, res / U.EPS64
+    _, xref, _ = ora.posv(uplo, A, B, nb)
+    assert np.abs(b - xref).max() <= 1e-10 * np.abs(xref).max()
+
+
+@pytest.mark.parametrize("uplo", [L, Up])
+def test_dpotrs(plasma, nb_guard, uplo):
+    n, nb, nrhs = 777, 128, 9
+    pa.set_param(pa.PlasmaTuning, pa.PlasmaDisabled)
+    pa.set_param(pa.PlasmaNb, nb)
+    A = U.tester_spd(n)
+    f = A.copy(order="F")
+    assert pa.potrf(uplo, n, f, n) == 0
+    B = U.rhs(n, nrhs, seed=5)
+    b = B.copy(order="F")
+    assert pa.potrs(uplo, n, nrhs, f, n, b, n) == 0
+    assert U.scaled_residual(A, b, B) < 10 * U.EPS64
+
+
+@pytest.mark.parametrize("uplo", [L, Up])
+@pytest.mark.parametrize("zerocol", [0, 37, 255, 256, 299])
+def test_dpotrf_not_pd(plasma, nb_guard, uplo, zerocol):
+    """zerocol (test/test_zpotrf.c:97-99): info = LAPACK's, pA unchanged."""
+    import scipy.linalg as sl
+    n, nb = 300, 128
+    pa.set_param(pa.PlasmaTuning, pa.PlasmaDisabled)
+    pa.set_param(pa.PlasmaNb, nb)
+    A = U.tester_spd(n)
+    A[:, zerocol] = 0.0
+    a = A.copy(order="F")
+    info = pa.potrf(uplo, n, a, n)
+    lapinfo = sl.lapack.dpotrf(A, lower=(uplo == L))[1]
+    assert info == lapinfo == zerocol + 1
+    assert np.array_equal(a, A)
+    _, oinfo = ora.potrf(uplo, A, nb)
+    assert oinfo == info
+
+
+def test_argument_codes(plasma):
+    a = np.eye(4, order="F")
+    b = np.ones((4, 2), order="F")
+    assert pa.potrf(999, 4, a, 4) == -1
+    assert pa.potrf(L, -1, a, 4) == -2
+    assert pa.potrf(L, 4, a, 3) == -4
+    assert pa.posv(999, 4, 2, a, 4, b, 4) == -1
+    assert pa.posv(L, -1, 2, a, 4, b, 4) == -2
+    assert pa.posv(L, 4, -1, a, 4, b, 4) == -3
+    assert pa.posv(L, 4, 2, a, 3, b, 4) == -5
+    assert pa.posv(L, 4, 2, a, 4, b, 3) == -7
+    assert pa.potrs(L, 4, 2, a, 4, b, 3) == -7
+    # quick returns
+    assert pa.potrf(L, 0, a, 1) == 0
+    assert pa.posv(L, 4, 0, a, 4, b, 4) == 0
+
+
+@pytest.mark.parametrize("uplo", [L, Up])
+@pytest.mark.parametrize("n,nb", [(100, 16), (1000, 256), (1536, 512)])
+def test_spotrf_sposv(plasma, nb_guard, uplo, n, nb):
+    pa.set_param(pa.PlasmaTuning, pa.PlasmaDisabled)
+    pa.set_param(pa.PlasmaNb, nb)
+    A = U.ggt_spd(n, seed=7, dtype=np.float32)
+    a = A.copy(order="F")
+    assert pa.potrf(uplo, n, a, n, prec="s") == 0
+    ref, _ = ora.potrf(uplo, A, nb, dtype=np.float32)
+    assert U.factor_error(a, ref, uplo == L) <= 1e-5
+    B = U.rhs(n, 8, seed=3, dtype=np.float32)
+    a2, b = A.copy(order="F"), B.copy(order="F")
+    assert pa.posv(uplo, n, 8, a2, n, b, n, prec="s") == 0
+    res = U.scaled_residual(A.astype(np.float64), b.astype(np.float64),
+                            B.astype(np.float64))
+    assert res < 10 * U.EPS32
+
+
+@pytest.mark.parametrize("lookahead", [0, 1])
+def test_dposv_8192(plasma, nb_guard, lookahead):
+    """A full-size-class solve on device memory: residual and the factor
+    against the oracle (MKL tiles on the host)."""
+    import torch
+    n, nb, nrhs = 8192, 256, 64
+    pa.set_param(pa.PlasmaTuning, pa.PlasmaDisabled)
+    pa.set_param(pa.PlasmaNb, nb)
+    pa.set_param(pa.PlasmaLookahead, lookahead)
+    A = U.ggt_spd(n, seed=11)
+    B = U.rhs(n, nrhs, seed=12)
+    da = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()
+    db = torch.from_numpy(np.ascontiguousarray(B.T)).cuda()
+    torch.cuda.synchronize()
+    assert pa.posv(L, n, nrhs, da, n, db, n) == 0
+    X = db.cpu().numpy().T
+    assert U.scaled_residual(A, X, B) < 10 * U.EPS64
+    F = da.cpu().numpy().T
+    ref, _ = _oracle_factor(L, A, nb)
+    assert U.factor_error(F, ref, True) <= 1e-12

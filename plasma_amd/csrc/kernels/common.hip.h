@@ -2,10 +2,13 @@
 //
 //  * Mfma<T>: one wave computes a 16x16 block D += A(16x4) * B(4x16) with
 //    v_mfma_f64_16x16x4_f64 (T = double) or v_mfma_f32_16x16x4_f32
-//    (T = float).  Operand/result lane layout (both instructions):
+//    (T = float).  Operand lane layout (both instructions):
 //      A: lane l holds A[l % 16][l / 16]
 //      B: lane l holds B[l / 16][l % 16]
-//      D: lane l, element q holds D[4 * (l / 16) + q][l % 16]
+//    Result: lane l, element q holds D[row][l % 16] with
+//      row = l / 16 + 4 * q       for the f64 instruction,
+//      row = 4 * (l / 16) + q     for the f32 instruction
+//    (both measured on gfx950 with an in-bounds probe).
 //  * View<T>: a strided 2-D window, element (r, c) at p[r * rs + c * cs].
 //    lower_view() returns tile (m, n) of the lower-equivalent factor
 //    L = (stored triangle) for PlasmaLower, or U^T for PlasmaUpper, so that
@@ -29,6 +32,10 @@ template <> struct Mfma<double> {
     {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int q)
+    {
+        return lane / 16 + 4 * q;
+    }
 };
 
 template <> struct Mfma<float> {
@@ -36,6 +43,10 @@ template <> struct Mfma<float> {
     static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c)
     {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int q)
+    {
+        return 4 * (lane / 16) + q;
     }
 };
 
@@ -248,7 +259,7 @@ struct GemmNT {
     static __device__ __forceinline__ int acc_row(int i, int q)
     {
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        return (wave / (BN / WN)) * WM + i * 16 + 4 * (lane / 16) + q;
+        return (wave / (BN / WN)) * WM + i * 16 + Mfma<T>::row(lane, q);
     }
     static __device__ __forceinline__ int acc_col(int j)
     {

@@ -62,6 +62,8 @@ __global__ __launch_bounds__(NTHREADS) void lag2_kernel(
 {
     if (block_should_stop(status)) return;
     const int m = blockIdx.x % A.mt, n = blockIdx.x / A.mt;
+    // triangular descriptors hold only the tiles of their triangle
+    if (!tile_in_region(A.type, m, n)) return;
     const int mv = plasma_tile_mview(A, m), nv = plasma_tile_nview(A, n);
     const S *a = (const S *)plasma_tile_addr(A, m, n);
     D *b = (D *)plasma_tile_addr(B, m, n);

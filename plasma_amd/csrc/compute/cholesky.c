@@ -116,7 +116,7 @@ void plasma_omp_xpotrf(plasma_enum_t precision, plasma_enum_t uplo,
         plasma_request_fail(sequence, request, PlasmaErrorIllegalValue);
         return;
     }
-    plasma_pxpotrf(uplo, A, sequence, request);
+    plasma_pxpotrf(uplo, A, NULL, sequence, request);
 }
 
 static int check_pair(plasma_enum_t precision, plasma_desc_t A,
@@ -167,7 +167,7 @@ void plasma_omp_xpotrs(plasma_enum_t precision, plasma_enum_t uplo,
         return;
     if (A.n == 0 || B.n == 0)
         return;
-    plasma_pxpotrs(uplo, A, B, sequence, request);
+    plasma_pxpotrs(uplo, A, B, NULL, sequence, request);
 }
 
 void plasma_omp_xposv(plasma_enum_t precision, plasma_enum_t uplo,
@@ -187,8 +187,16 @@ void plasma_omp_xposv(plasma_enum_t precision, plasma_enum_t uplo,
         return;
     if (A.n == 0 || B.n == 0)
         return;
-    plasma_pxpotrf(uplo, A, sequence, request);
-    plasma_pxpotrs(uplo, A, B, sequence, request);
+    /* the factorization keeps the inverses of its diagonal tiles for the
+     * two sweeps */
+    void *inv = plasma_dev_alloc(plasma_inv_bytes(A));
+    if (inv == NULL) {
+        plasma_request_fail(sequence, request, PlasmaErrorOutOfMemory);
+        return;
+    }
+    plasma_pxpotrf(uplo, A, inv, sequence, request);
+    plasma_pxpotrs(uplo, A, B, inv, sequence, request);
+    plasma_dev_free(inv);
 }
 
 /***************************************************************************/

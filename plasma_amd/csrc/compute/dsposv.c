@@ -122,6 +122,13 @@ void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
         plasma_request_fail(sequence, request, PlasmaErrorOutOfMemory);
         return;
     }
+    void *inv_s = plasma_dev_alloc(plasma_inv_bytes(As));
+    if (inv_s == NULL) {
+        plasma_dev_free(work);
+        plasma_request_fail(sequence, request, PlasmaErrorOutOfMemory);
+        return;
+    }
+    void *inv_d = NULL;
     double *xnorm = work + (size_t)A.mt * A.mb;
     double *rnorm = xnorm + nrhs;
     double *anorm = rnorm + nrhs;
@@ -136,8 +143,8 @@ void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
         issue(plasma_kern_lag2(B, Xs, st, s), sequence, request) ||
         issue(plasma_kern_lag2(A, As, st, s), sequence, request))
         goto done;
-    plasma_pxpotrf(uplo, As, sequence, request);
-    plasma_pxpotrs(uplo, As, Xs, sequence, request);
+    plasma_pxpotrf(uplo, As, inv_s, sequence, request);
+    plasma_pxpotrs(uplo, As, Xs, inv_s, sequence, request);
     if (issue(plasma_kern_lag2(Xs, X, st, s), sequence, request))
         goto done;
 
@@ -158,7 +165,7 @@ void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
     for (int it = 0; it < itermax; it++) {
         if (issue(plasma_kern_lag2(R, Xs, st, s), sequence, request))
             goto done;
-        plasma_pxpotrs(uplo, As, Xs, sequence, request);
+        plasma_pxpotrs(uplo, As, Xs, inv_s, sequence, request);
         if (issue(plasma_kern_lag2(Xs, R, st, s), sequence, request) ||
             issue(plasma_kern_geadd(1.0, R, 1.0, X, st, s), sequence,
                   request) ||
@@ -184,12 +191,19 @@ void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
 
     /* no convergence: FP64 factorization (reference :524-538) */
     *iter = -itermax - 1;
-    plasma_pxpotrf(uplo, A, sequence, request);
+    inv_d = plasma_dev_alloc(plasma_inv_bytes(A));
+    if (inv_d == NULL) {
+        plasma_request_fail(sequence, request, PlasmaErrorOutOfMemory);
+        goto done;
+    }
+    plasma_pxpotrf(uplo, A, inv_d, sequence, request);
     if (issue(plasma_kern_lacpy(B, X, st, s), sequence, request))
         goto done;
-    plasma_pxpotrs(uplo, A, X, sequence, request);
+    plasma_pxpotrs(uplo, A, X, inv_d, sequence, request);
 
 done:
+    plasma_dev_free(inv_d);
+    plasma_dev_free(inv_s);
     plasma_dev_free(work);
 }
 

@@ -1,15 +1,15 @@
 /*
- * pxpotrf.c -- tile Cholesky as a lookahead-scheduled, stream-ordered
- * launch sequence.
+ * pxpotrf.c -- tile Cholesky and its triangular sweeps as lookahead-
+ * scheduled, stream-ordered launch sequences.
  *
  * The reference inserts one OpenMP task per tile operation and lets the
  * runtime order them by their depend clauses (compute/pzpotrf.c:37-80 for
  * lower, :84-126 for upper; 341 376 gemm tasks at n = 32768, nb = 256).
- * Here every step k issues three launches that each cover all tiles of one
- * task class:
- *     potrf(k)            diagonal tile                 (core_zpotrf)
- *     trsm(k)             panel tiles m > k             (core_ztrsm)
- *     update(k, n0, n1)   trailing tiles, columns n0..n1 (core_zherk/zgemm)
+ * Here every step k issues one launch per task class covering all tiles of
+ * that step:
+ *     potrf(k)            diagonal tile, plus W_kk = L(k,k)^{-1}
+ *     trsm(k)             panel tiles m > k: A(m,k) W_kk^T (MFMA)
+ *     update(k, n0, n1)   trailing tiles of columns n0..n1 (MFMA)
  * With lookahead depth >= 1 the update of step k is split: column k+1 is
  * updated first on the main stream, the panel stream then factors column
  * k+1 (potrf + trsm) while the main stream updates the remaining columns.
@@ -27,6 +27,11 @@ static int issue(int err, plasma_sequence_t *sequence,
     if (err != 0)
         plasma_request_fail(sequence, request, err);
     return err;
+}
+
+size_t plasma_inv_bytes(plasma_desc_t A)
+{
+    return (size_t)A.mt * A.mb * A.mb * plasma_element_size(A.precision);
 }
 
 /* Algorithmic flops of the launches (reference test/flops.h conventions:
@@ -49,30 +54,30 @@ static double update_flops(plasma_desc_t A, int n0, int n1)
     double f = 0, nb = A.nb;
     for (int n = n0; n < n1 && n < A.mt; n++) {
         double nv = plasma_tile_mview(A, n);
-        f += nb * nv * (nv + 1);                       /* syrk on (n, n) */
+        f += nb * nv * (nv + 1);                           /* syrk (n, n) */
         for (int m = n + 1; m < A.mt; m++)
             f += 2.0 * plasma_tile_mview(A, m) * nv * nb;  /* gemm */
     }
     return f;
 }
 
-static int k_potrf(plasma_enum_t uplo, plasma_desc_t A, int k, int *st,
-                   hipStream_t s)
+static int k_potrf(plasma_enum_t uplo, plasma_desc_t A, int k, void *inv,
+                   int *st, hipStream_t s)
 {
     int slot;
     plasma_prof_begin(s, PlasmaProfPotrf, potrf_flops(plasma_tile_mview(A, k)),
                       0, &slot);
-    int err = plasma_kern_potrf_diag(uplo, A, k, st, s);
+    int err = plasma_kern_potrf_diag(uplo, A, k, inv, st, s);
     plasma_prof_end(s, slot);
     return err;
 }
 
-static int k_panel(plasma_enum_t uplo, plasma_desc_t A, int k, int *st,
-                   hipStream_t s)
+static int k_panel(plasma_enum_t uplo, plasma_desc_t A, int k, void *inv,
+                   void *scratch, int *st, hipStream_t s)
 {
     int slot;
     plasma_prof_begin(s, PlasmaProfPanel, panel_flops(A, k), 0, &slot);
-    int err = plasma_kern_trsm_panel(uplo, A, k, st, s);
+    int err = plasma_kern_trsm_panel(uplo, A, k, inv, scratch, st, s);
     plasma_prof_end(s, slot);
     return err;
 }
@@ -89,7 +94,7 @@ static int k_update(plasma_enum_t uplo, plasma_desc_t A, int k, int n0,
     return err;
 }
 
-void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A,
+void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A, void *inv,
                     plasma_sequence_t *sequence, plasma_request_t *request)
 {
     if (sequence->status != PlasmaSuccess)
@@ -98,16 +103,35 @@ void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A,
     int *st = plasma_sequence_dstatus(sequence, request);
     hipStream_t s0 = ctx->stream, s1 = ctx->panel_stream;
     const int mt = A.mt;
+    size_t es = plasma_element_size(A.precision);
+
+    void *own_inv = NULL;
+    if (inv == NULL) {
+        inv = own_inv = plasma_dev_alloc(plasma_inv_bytes(A));
+        if (inv == NULL) {
+            plasma_request_fail(sequence, request, PlasmaErrorOutOfMemory);
+            return;
+        }
+    }
+    void *scratch = plasma_dev_alloc((size_t)imax(mt - 1, 1) * A.mb * A.nb *
+                                     es);
+    if (scratch == NULL) {
+        plasma_dev_free(own_inv);
+        plasma_request_fail(sequence, request, PlasmaErrorOutOfMemory);
+        return;
+    }
 
     if (ctx->lookahead == 0 || mt < 3) {
         for (int k = 0; k < mt; k++) {
-            if (issue(k_potrf(uplo, A, k, st, s0),
-                      sequence, request) ||
-                issue(k_panel(uplo, A, k, st, s0),
-                      sequence, request) ||
-                issue(k_update(uplo, A, k, k + 1, mt, st, s0), sequence, request))
-                return;
+            if (issue(k_potrf(uplo, A, k, inv, st, s0), sequence, request) ||
+                issue(k_panel(uplo, A, k, inv, scratch, st, s0), sequence,
+                      request) ||
+                issue(k_update(uplo, A, k, k + 1, mt, st, s0), sequence,
+                      request))
+                break;
         }
+        plasma_dev_free(scratch);
+        plasma_dev_free(own_inv);
         return;
     }
 
@@ -122,6 +146,8 @@ void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A,
                                                  hipEventDisableTiming),
                          "hipEventCreate")) {
         plasma_request_fail(sequence, request, PlasmaErrorInternal);
+        plasma_dev_free(scratch);
+        plasma_dev_free(own_inv);
         return;
     }
 
@@ -129,10 +155,9 @@ void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A,
     hipEventRecord(ev_fork, s0);
     hipStreamWaitEvent(s1, ev_fork, 0);
 
-    int err = 0;
-    err = issue(k_potrf(uplo, A, 0, st, s1), sequence, request);
+    int err = issue(k_potrf(uplo, A, 0, inv, st, s1), sequence, request);
     if (!err)
-        err = issue(k_panel(uplo, A, 0, st, s1), sequence,
+        err = issue(k_panel(uplo, A, 0, inv, scratch, st, s1), sequence,
                     request);
     hipEventRecord(ev_panel, s1);
 
@@ -140,16 +165,16 @@ void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A,
         /* main stream: wait for panel k, update column k+1 */
         hipStreamWaitEvent(s0, ev_panel, 0);
         if (k + 1 < mt) {
-            err = issue(k_update(uplo, A, k, k + 1, k + 2, st, s0),
-                        sequence, request);
+            err = issue(k_update(uplo, A, k, k + 1, k + 2, st, s0), sequence,
+                        request);
             hipEventRecord(ev_col, s0);
             /* panel stream: factor column k+1 */
             hipStreamWaitEvent(s1, ev_col, 0);
             if (!err)
-                err = issue(k_potrf(uplo, A, k + 1, st, s1),
-                            sequence, request);
+                err = issue(k_potrf(uplo, A, k + 1, inv, st, s1), sequence,
+                            request);
             if (!err)
-                err = issue(k_panel(uplo, A, k + 1, st, s1),
+                err = issue(k_panel(uplo, A, k + 1, inv, scratch, st, s1),
                             sequence, request);
             hipEventRecord(ev_panel, s1);
             /* main stream: the rest of the trailing matrix */
@@ -164,6 +189,8 @@ void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A,
     hipEventDestroy(ev_fork);
     hipEventDestroy(ev_col);
     hipEventDestroy(ev_panel);
+    plasma_dev_free(scratch);
+    plasma_dev_free(own_inv);
 }
 
 /*
@@ -172,7 +199,7 @@ void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A,
  * reduce to a forward sweep with L or a backward sweep with L^T.
  */
 void plasma_pxtrsm_left(plasma_enum_t uplo, plasma_enum_t trans,
-                        plasma_desc_t A, plasma_desc_t B,
+                        plasma_desc_t A, plasma_desc_t B, const void *inv,
                         plasma_sequence_t *sequence,
                         plasma_request_t *request)
 {
@@ -185,6 +212,12 @@ void plasma_pxtrsm_left(plasma_enum_t uplo, plasma_enum_t trans,
     const int backward = (uplo == PlasmaLower) ? (trans != PlasmaNoTrans)
                                                : (trans == PlasmaNoTrans);
     const int mt = B.mt;
+    size_t es = plasma_element_size(B.precision);
+    void *scratch = plasma_dev_alloc((size_t)A.mb * B.nt * B.nb * es);
+    if (scratch == NULL) {
+        plasma_request_fail(sequence, request, PlasmaErrorOutOfMemory);
+        return;
+    }
     for (int i = 0; i < mt; i++) {
         const int k = backward ? mt - 1 - i : i;
         int slot;
@@ -194,26 +227,47 @@ void plasma_pxtrsm_left(plasma_enum_t uplo, plasma_enum_t trans,
                           nbk * nbk * B.n + 2.0 * nbk * nupd * B.mb * B.n, 0,
                           &slot);
         int err = issue(plasma_kern_trsm_diag_left(uplo, backward, A, B, k,
-                                                   st, s0),
+                                                   inv, scratch, st, s0),
                         sequence, request);
         if (!err)
-            err = issue(plasma_kern_sweep_update(uplo, backward, A, B, k, st,
-                                                 s0),
+            err = issue(plasma_kern_sweep_update(uplo, backward, A, B, k,
+                                                 scratch, st, s0),
                         sequence, request);
         plasma_prof_end(s0, slot);
         if (err)
-            return;
+            break;
     }
+    plasma_dev_free(scratch);
 }
 
 void plasma_pxpotrs(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
-                    plasma_sequence_t *sequence, plasma_request_t *request)
+                    const void *inv, plasma_sequence_t *sequence,
+                    plasma_request_t *request)
 {
+    if (sequence->status != PlasmaSuccess)
+        return;
+    void *own_inv = NULL;
+    if (inv == NULL) {
+        plasma_context_t *ctx = plasma_context_self();
+        int *st = plasma_sequence_dstatus(sequence, request);
+        own_inv = plasma_dev_alloc(plasma_inv_bytes(A));
+        if (own_inv == NULL) {
+            plasma_request_fail(sequence, request, PlasmaErrorOutOfMemory);
+            return;
+        }
+        if (issue(plasma_kern_trtri_diag(uplo, A, own_inv, st, ctx->stream),
+                  sequence, request)) {
+            plasma_dev_free(own_inv);
+            return;
+        }
+        inv = own_inv;
+    }
     /* reference compute/zpotrs.c:270-283 */
     plasma_pxtrsm_left(uplo, uplo == PlasmaUpper ? PlasmaConjTrans
                                                  : PlasmaNoTrans,
-                       A, B, sequence, request);
+                       A, B, inv, sequence, request);
     plasma_pxtrsm_left(uplo, uplo == PlasmaUpper ? PlasmaNoTrans
                                                  : PlasmaConjTrans,
-                       A, B, sequence, request);
+                       A, B, inv, sequence, request);
+    plasma_dev_free(own_inv);
 }

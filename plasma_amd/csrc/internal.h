@@ -113,14 +113,21 @@ int plasma_is_device_ptr(const void *p);
  * Parallel tile algorithms (compute/p*.c).  Each enqueues its launches on
  * the context streams and returns; the main stream is the join point.
  */
-void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A,
+/* inv: the diagonal-tile inverses (mt * mb * mb elements of A's
+ * precision).  pxpotrf writes them when inv != NULL (and uses a temporary
+ * buffer otherwise); pxpotrs reads them, computing them first when
+ * inv == NULL. */
+void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A, void *inv,
                     plasma_sequence_t *sequence, plasma_request_t *request);
 void plasma_pxpotrs(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
-                    plasma_sequence_t *sequence, plasma_request_t *request);
+                    const void *inv, plasma_sequence_t *sequence,
+                    plasma_request_t *request);
 void plasma_pxtrsm_left(plasma_enum_t uplo, plasma_enum_t trans,
-                        plasma_desc_t A, plasma_desc_t B,
+                        plasma_desc_t A, plasma_desc_t B, const void *inv,
                         plasma_sequence_t *sequence,
                         plasma_request_t *request);
+/* bytes of the inverse workspace for A */
+size_t plasma_inv_bytes(plasma_desc_t A);
 
 /* layout translation between a LAPACK array (host or device) and tiles */
 void plasma_pxlapack2desc(plasma_enum_t region, const void *pA, int lda,

@@ -40,16 +40,25 @@ int plasma_kern_tile2lapack(plasma_enum_t region, plasma_desc_t A, void *pA,
 
 /*
  * Cholesky factorization of diagonal tile k (reference core_zpotrf.c:54-82
- * called from pzpotrf.c:41-45).  On a non positive (or NaN) pivot at local
- * column j the status word is set to nb*k + j + 1 if it was 0.
+ * called from pzpotrf.c:41-45), which also writes W_kk = L(k,k)^{-1} (lower,
+ * zeros above, ld = A.mb) to inv + k * A.mb * A.mb.  On a non positive (or
+ * NaN) pivot at local column j the status word is set to nb*k + j + 1 if it
+ * was 0.
  */
 int plasma_kern_potrf_diag(plasma_enum_t uplo, plasma_desc_t A, int k,
-                           int *status, hipStream_t s);
+                           void *inv, int *status, hipStream_t s);
 
-/* Panel solve A(m,k) <- A(m,k) L(k,k)^{-T} for m > k
- * (reference pzpotrf.c:47-57; upper: :90-99). */
-int plasma_kern_trsm_panel(plasma_enum_t uplo, plasma_desc_t A, int k,
+/* W_kk = L(k,k)^{-1} for every diagonal tile of an already factored A, one
+ * workgroup per tile (used by potrs without a preceding factorization). */
+int plasma_kern_trtri_diag(plasma_enum_t uplo, plasma_desc_t A, void *inv,
                            const int *status, hipStream_t s);
+
+/* Panel solve A(m,k) <- A(m,k) L(k,k)^{-T} = A(m,k) W_kk^T for m > k
+ * (reference pzpotrf.c:47-57; upper: :90-99), as an MFMA product into
+ * `scratch` ((mt-1) * mb * nb elements) followed by a copy back. */
+int plasma_kern_trsm_panel(plasma_enum_t uplo, plasma_desc_t A, int k,
+                           const void *inv, void *scratch, const int *status,
+                           hipStream_t s);
 
 /* Trailing update A(m,n) -= A(m,k) A(n,k)^T for n in [n_first, n_end) and
  * m in [n, mt), lower triangle only on diagonal tiles
@@ -61,16 +70,19 @@ int plasma_kern_trailing_update(plasma_enum_t uplo, plasma_desc_t A, int k,
 /*
  * Triangular sweeps of potrs (reference pztrsm.c:113-177 and :43-107).
  * backward == 0: solve L Y = B (forward); backward == 1: L^T X = Y.
- * trsm_diag solves with diagonal tile k on every tile column of B(k,:);
- * sweep_update subtracts the contribution of block row k from the block rows
- * still to be solved (m > k forward, m < k backward).
+ * trsm_diag_left computes S = W_kk B(k,:) (forward) or W_kk^T B(k,:)
+ * (backward) into `scratch` (mb * B.nt * nb elements, ld = mb);
+ * sweep_update subtracts L(m,k) S from the block rows still to be solved
+ * (m > k forward, m < k backward) and copies S into B(k,:).
  */
 int plasma_kern_trsm_diag_left(plasma_enum_t uplo, int backward,
                                plasma_desc_t A, plasma_desc_t B, int k,
+                               const void *inv, void *scratch,
                                const int *status, hipStream_t s);
 int plasma_kern_sweep_update(plasma_enum_t uplo, int backward,
                              plasma_desc_t A, plasma_desc_t B, int k,
-                             const int *status, hipStream_t s);
+                             const void *scratch, const int *status,
+                             hipStream_t s);
 
 /*
  * Mixed-precision helpers (reference compute/zcposv.c:363-539).

@@ -1,21 +1,34 @@
 #!/usr/bin/env python3
 """Benchmark of the Cholesky solve path on one MI355X.
 
-Workload (the headline metric's configuration): plasma_dposv, n = 32768,
-nrhs = 64, lower, A = G G^T / n + I with G iid N(0,1) (torch generator seed
-0), B iid N(0,1).  Inputs are generated on the GPU and stay resident in HBM.
-One step is exactly the work of plasma_dposv on device-resident LAPACK
-arrays: tile descriptors from the pool, tr2desc(A) + ge2desc(B), tile
-Cholesky + two triangular sweeps, desc2tr / desc2ge into output arrays,
-then the sequence is synchronised.  The input arrays are only read, so every
-step factors the same matrix.
+Headline metric: time to an FP64 backward error for A X = B at n = 32768,
+nrhs = 64, mixed-precision plasma_dsposv (FP32 tile Cholesky on MFMA, FP64
+iterative refinement) against plain FP64 plasma_dposv, both reported as
+FP64-equivalent TFLOP/s, flops_dpotrf(n) + flops_dpotrs(n, nrhs) per solve
+(the reference tester's convention, test/flops.h:366-417 and
+test/test_zcposv.c:314).
 
-Printed JSON: TFLOP/s with the reference tester's flop count
-flops_dpotrf(n) + flops_dpotrs(n, nrhs) (test/flops.h:366-417), the scaled
-residual of the last solution, the trailing-update kernel's roofline
-(HIP events on its own stream) and the CPU baseline: the oracle (PLASMA's
-tile algorithm restated, OpenMP tasks, sequential MKL tile kernels) timed
-on this host on a smaller n.
+Matrices (synthetic, generated on the GPU, resident in HBM):
+  * A = Q diag(s) Q^T, Q the product of 64 Householder reflectors with
+    N(0,1) vectors, s log-spaced on [1/kappa, 1] (so ||A||_2 = 1 and
+    cond_2(A) = kappa), kappa in {1e2, 1e4, 1e6}; the headline value is the
+    mixed solve at the largest kappa;
+  * A = G G^T / n + I (G iid N(0,1)) for the plain FP64 drop-in line.
+B is iid N(0,1).  Every solve is checked afterwards with the tester's
+scaled residual ||B - A X||_inf / (n ||A||_inf ||X||_inf eps64) < 10.
+
+One mixed step is one plasma_dsposv call on device-resident LAPACK arrays
+(translation to tiles, FP32 factor, refinement, translation back, final
+synchronisation).  One plain step is the same work as plasma_dposv on
+device-resident arrays (tr2desc + ge2desc, tile Cholesky, two sweeps,
+desc2tr + desc2ge); the input arrays are only read, so every step solves
+the same system.
+
+Also printed: the dominant kernel's roofline (the FP32 trailing update of
+the mixed solve; HIP events on the stream it runs on), per-kernel-class
+times, and the CPU baseline: the oracle's dsposv (PLASMA's tile algorithm
+restated, OpenMP tasks, sequential MKL tile kernels) timed on this host on
+a smaller n.
 """
 import argparse
 import ctypes
@@ -28,7 +41,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_MFMA_PEAK = 78.6e12      # MI355X dense FP64 matrix peak (FLOP/s)
+FP32_MFMA_PEAK = 157.3e12     # MI355X dense FP32 matrix peak (FLOP/s)
 HBM_PEAK = 8.0e12             # bytes/s
+EPS64 = 2.0 ** -53
+
+PROF_CLASSES = ["trailing_update", "potrf_diag", "trsm_panel", "sweeps",
+                "layout", "mixed"]
 
 
 def flops_posv(n, nrhs):
@@ -36,6 +54,64 @@ def flops_posv(n, nrhs):
     potrf = n ** 3 / 3.0 + n ** 2 / 2.0 + n / 6.0
     potrs = 2.0 * n * n * nrhs
     return potrf + potrs
+
+
+def householder_spd(n, kappa, nrefl=64, seed=0, device="cuda"):
+    """A = Q diag(s) Q^T with Q = H_1 ... H_nrefl, H_i = I - tau_i v_i v_i^T,
+    v_i iid N(0,1) (torch generator `seed`), s_i = kappa^(-i/(n-1)).
+
+    Q is applied in compact WY form, Q = I - V T V^T (T upper triangular,
+    the forward recurrence T(1:i-1, i) = -tau_i T(1:i-1, 1:i-1) V^T v_i), so
+    A = D + W M W^T with W = [V, D V] and M = [[C, -T], [-T^T, 0]],
+    C = T (V^T D V) T^T: one rank-2*nrefl update of the diagonal.  The
+    result is symmetrised exactly.  Returns (A, s) as float64 tensors."""
+    import torch
+    f64 = torch.float64
+    g = torch.Generator(device=device).manual_seed(seed)
+    V = torch.randn(nrefl, n, dtype=f64, device=device, generator=g).T
+    i = torch.arange(n, dtype=f64, device=device)
+    s = torch.pow(torch.tensor(float(kappa), dtype=f64, device=device),
+                  -i / max(n - 1, 1))
+    gram = (V.T @ V).cpu()
+    tau = 2.0 / gram.diagonal()
+    T = torch.zeros(nrefl, nrefl, dtype=f64)
+    for j in range(nrefl):
+        T[j, j] = tau[j]
+        if j:
+            T[:j, j] = -tau[j] * (T[:j, :j] @ gram[:j, j])
+    U = V * s[:, None]
+    C = T @ (V.T @ U).cpu() @ T.T
+    M = torch.zeros(2 * nrefl, 2 * nrefl, dtype=f64)
+    M[:nrefl, :nrefl] = C
+    M[:nrefl, nrefl:] = -T
+    M[nrefl:, :nrefl] = -T.T
+    W = torch.cat([V, U], dim=1)
+    A = (W @ M.to(device)) @ W.T
+    del W, U, V
+    A.diagonal().add_(s)
+    A = (A + A.T).mul_(0.5)
+    return A, s
+
+
+def ggt_spd(n, seed=0, device="cuda"):
+    """A = G G^T / n + I, G iid N(0,1) (torch generator `seed`)."""
+    import torch
+    g = torch.Generator(device=device).manual_seed(seed)
+    G = torch.randn(n, n, dtype=torch.float64, device=device, generator=g)
+    A = (G @ G.T).div_(n)
+    A.diagonal().add_(1.0)
+    return A
+
+
+def scaled_residual(A, X, B):
+    """||B - A X||_inf / (n ||A||_inf ||X||_inf eps64); X, B column-major
+    n x nrhs stored as (nrhs, n) tensors, A symmetric."""
+    n = A.shape[0]
+    R = B.T - A @ X.T
+    an = A.abs().sum(1).max().item()
+    xn = X.T.abs().sum(1).max().item()
+    rn = R.abs().sum(1).max().item()
+    return rn / (n * an * xn) / EPS64
 
 
 def parse():
@@ -47,34 +123,38 @@ def parse():
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--nb", type=int, default=256)
     ap.add_argument("--lookahead", type=int, default=1)
+    ap.add_argument("--kappa", type=str, default="1e2,1e4,1e6",
+                    help="condition numbers of the mixed-precision sweep; "
+                         "the headline value is the last one")
+    ap.add_argument("--no-ggt", action="store_true",
+                    help="skip the plain FP64 line on A = G G^T/n + I")
     ap.add_argument("--cpu-n", type=int, default=16384,
                     help="order of the CPU-baseline sample (0 skips it)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
 
 
-def cpu_baseline(n, nrhs, nb, threads):
-    """Oracle dposv on the host: median of 2 runs after one warm-up."""
+def cpu_baseline(n, nrhs, nb, kappa, threads):
+    """Oracle dsposv on the host on the same kind of matrix (Householder,
+    cond = kappa) at order n: median of 2 runs after one warm-up."""
     import numpy as np
-    import torch
     import oracle as ora
     backend = ora.set_blas(ora.MKL)
     if threads <= 0:
         threads = min(16, len(os.sched_getaffinity(0)))
     ora.set_threads(threads)
-    g = torch.Generator(device="cuda").manual_seed(1)
-    G = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
-    A = (G @ G.T).div_(n)
-    A.diagonal().add_(1.0)
+    import torch
+    A, _ = householder_spd(n, kappa, seed=1)
+    g = torch.Generator(device="cuda").manual_seed(2)
     B = torch.randn(nrhs, n, dtype=torch.float64, device="cuda", generator=g)
     A = np.asfortranarray(A.cpu().numpy())
     B = np.asfortranarray(B.cpu().numpy().T)
-    del G
-    times = []
-    for i in range(3):
+    times, iters = [], []
+    for _ in range(3):
         t0 = time.perf_counter()
-        _, X, info = ora.posv(ora.LOWER, A, B, nb)
+        _, it, info = ora.dsposv(ora.LOWER, A, B, nb)
         times.append(time.perf_counter() - t0)
+        iters.append(it)
         assert info == 0
     t = sorted(times[1:])[len(times[1:]) // 2]
     return {
@@ -82,12 +162,143 @@ def cpu_baseline(n, nrhs, nb, threads):
         "unit": "TFLOP/s",
         "cores": threads,
         "kind": "port",
-        "sample": "oracle dposv n=%d nrhs=%d nb=%d lower, OpenMP tile tasks, "
-                  "%s sequential tile kernels, median of 2 runs after 1 "
-                  "warm-up (%.2f s per solve)"
-                  % (n, nrhs, nb, "MKL" if backend == ora.MKL else "built-in",
-                     t),
+        "sample": "oracle dsposv n=%d nrhs=%d nb=%d lower, Householder "
+                  "A cond %.0e, OpenMP tile tasks, %s sequential tile "
+                  "kernels, median of 2 runs after 1 warm-up (%.2f s per "
+                  "solve, %d refinement iterations)"
+                  % (n, nrhs, nb, kappa,
+                     "MKL" if backend == ora.MKL else "built-in", t,
+                     iters[-1]),
     }
+
+
+class Bench:
+    def __init__(self, args):
+        import torch
+        import plasma_amd as pa
+        self.torch, self.pa, self.args = torch, pa, args
+        self.lib = lib = pa.load()
+        rc = lib.plasma_init()
+        if rc != 0:
+            raise SystemExit("plasma_init failed: %d" % rc)
+        pa.set_param(pa.PlasmaTuning, pa.PlasmaDisabled)
+        pa.set_param(pa.PlasmaNb, args.nb)
+        pa.set_param(pa.PlasmaLookahead, args.lookahead)
+
+    def prof_reset(self):
+        self.lib.plasma_profile_enable(0)
+        self.lib.plasma_profile_enable(1)
+
+    def prof_read(self, steps):
+        lib = self.lib
+        out = {}
+        for cls, name in enumerate(PROF_CLASSES):
+            a, f, b, c = (ctypes.c_double(), ctypes.c_double(),
+                          ctypes.c_double(), ctypes.c_long())
+            lib.plasma_profile_query(cls, ctypes.byref(a), ctypes.byref(f),
+                                     ctypes.byref(b), ctypes.byref(c))
+            if c.value:
+                out[name] = {"ms": a.value, "flops": f.value,
+                             "bytes": b.value, "launches": c.value}
+        lib.plasma_profile_enable(0)
+        return out
+
+    def timed(self, step, label):
+        torch, args = self.torch, self.args
+        for _ in range(args.warmup):
+            step()
+        print("bench: %s warmup done" % label, file=sys.stderr, flush=True)
+        self.prof_reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            step()
+            print("bench: %s step %d done" % (label, i), file=sys.stderr,
+                  flush=True)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        return ms, self.prof_read(args.steps)
+
+    def plain(self, A0, B0, label):
+        """Plain FP64 dposv on device-resident LAPACK arrays."""
+        torch, pa, lib = self.torch, self.pa, self.lib
+        n, nrhs, nb = A0.shape[0], B0.shape[0], self.args.nb
+        Aout = torch.empty_like(A0)
+        X = torch.empty_like(B0)
+        A, B = pa.plasma_desc_t(), pa.plasma_desc_t()
+
+        def step():
+            assert lib.plasma_desc_triangular_create(
+                pa.PlasmaRealDouble, pa.PlasmaLower, nb, nb, n, n, 0, 0, n,
+                n, ctypes.byref(A)) == 0
+            assert lib.plasma_desc_general_create(
+                pa.PlasmaRealDouble, nb, nb, n, nrhs, 0, 0, n, nrhs,
+                ctypes.byref(B)) == 0
+            seq, req = pa.plasma_sequence_t(), pa.plasma_request_t()
+            lib.plasma_sequence_init(ctypes.byref(seq))
+            lib.plasma_request_init(ctypes.byref(req))
+            s, r = ctypes.byref(seq), ctypes.byref(req)
+            lib.plasma_omp_dtr2desc(A0.data_ptr(), n, A, s, r)
+            lib.plasma_omp_dge2desc(B0.data_ptr(), n, B, s, r)
+            lib.plasma_omp_dposv(pa.PlasmaLower, A, B, s, r)
+            lib.plasma_omp_ddesc2tr(A, Aout.data_ptr(), n, s, r)
+            lib.plasma_omp_ddesc2ge(B, X.data_ptr(), n, s, r)
+            status = lib.plasma_sequence_sync(s)
+            lib.plasma_desc_destroy(ctypes.byref(A))
+            lib.plasma_desc_destroy(ctypes.byref(B))
+            if status != 0:
+                raise SystemExit("plasma dposv failed: %d" % status)
+
+        ms, prof = self.timed(step, label)
+        del Aout
+        res = scaled_residual(A0, X, B0)
+        return ms, prof, res
+
+    def mixed(self, A0, B0, label):
+        """plasma_dsposv on device-resident LAPACK arrays."""
+        torch, pa, lib = self.torch, self.pa, self.lib
+        n, nrhs = A0.shape[0], B0.shape[0]
+        X = torch.empty_like(B0)
+        it = ctypes.c_int(0)
+        iters = []
+
+        def step():
+            rc = lib.plasma_dsposv(pa.PlasmaLower, n, nrhs, A0.data_ptr(), n,
+                                   B0.data_ptr(), n, X.data_ptr(), n,
+                                   ctypes.byref(it))
+            if rc != 0:
+                raise SystemExit("plasma dsposv failed: %d" % rc)
+            iters.append(it.value)
+
+        ms, prof = self.timed(step, label)
+        res = scaled_residual(A0, X, B0)
+        return ms, prof, res, iters[-1]
+
+
+def classes(prof, steps):
+    out = {}
+    for name, p in prof.items():
+        a, f, b = p["ms"], p["flops"], p["bytes"]
+        out[name] = {"ms_per_step": round(a / steps, 3),
+                     "launches_per_step": p["launches"] // steps,
+                     "tflops": round(f / (a * 1e-3) / 1e12, 2)
+                     if a and f else None,
+                     "gbps": round(b / (a * 1e-3) / 1e9, 1)
+                     if a and b else None}
+    return out
+
+
+def roofline(prof, kernel, peak):
+    p = prof.get("trailing_update")
+    if not p:
+        return None
+    achieved = p["flops"] / (p["ms"] * 1e-3)
+    return {"kernel": kernel, "bound": "mfma",
+            "achieved": round(achieved / 1e12, 3),
+            "peak": peak / 1e12, "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 4),
+            "avg_launch_ms": round(p["ms"] / p["launches"], 4),
+            "traffic": None}
 
 
 def main():
@@ -98,147 +309,97 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import torch
-    import plasma_amd as pa
-
-    lib = pa.load()
-    rc = lib.plasma_init()
-    if rc != 0:
-        raise SystemExit("plasma_init failed: %d" % rc)
-    pa.set_param(pa.PlasmaTuning, pa.PlasmaDisabled)
-    pa.set_param(pa.PlasmaNb, args.nb)
-    pa.set_param(pa.PlasmaLookahead, args.lookahead)
-    lib.plasma_profile_enable.argtypes = [ctypes.c_int]
-    lib.plasma_profile_query.argtypes = [
-        ctypes.c_int, ctypes.POINTER(ctypes.c_double),
-        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
-        ctypes.POINTER(ctypes.c_long)]
-
+    bench = Bench(args)
     n, nrhs, nb = args.n, args.nrhs, args.nb
-    gen = torch.Generator(device="cuda").manual_seed(0)
-    G = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=gen)
-    A0 = (G @ G.T).div_(n)
-    A0.diagonal().add_(1.0)
-    del G
-    # column-major n x nrhs: row j of the tensor is column j
-    B0 = torch.randn(nrhs, n, dtype=torch.float64, device="cuda", generator=gen)
-    Aout = torch.empty_like(A0)
-    X = torch.empty_like(B0)
-    torch.cuda.synchronize()
+    fl = flops_posv(n, nrhs)
+    kappas = [float(k) for k in args.kappa.split(",") if k]
+    g = torch.Generator(device="cuda").manual_seed(1)
+    B0 = torch.randn(nrhs, n, dtype=torch.float64, device="cuda", generator=g)
 
-    A = pa.plasma_desc_t()
-    B = pa.plasma_desc_t()
+    sweep = {}
+    head = None
+    for kappa in kappas:
+        A0, _ = householder_spd(n, kappa, seed=0)
+        torch.cuda.synchronize()
+        tag = "kappa=%.0e" % kappa
+        ms_m, prof_m, res_m, iters = bench.mixed(A0, B0, "mixed " + tag)
+        ms_p, prof_p, res_p = bench.plain(A0, B0, "plain " + tag)
+        del A0
+        sweep[tag] = {
+            "mixed_tflops": round(fl / (ms_m * 1e-3) / 1e12, 3),
+            "mixed_ms": round(ms_m, 3),
+            "iterations": iters,
+            "mixed_residual_eps": round(res_m, 4),
+            "plain_tflops": round(fl / (ms_p * 1e-3) / 1e12, 3),
+            "plain_ms": round(ms_p, 3),
+            "plain_residual_eps": round(res_p, 4),
+            "speedup": round(ms_p / ms_m, 3),
+        }
+        head = (kappa, ms_m, prof_m, res_m, iters, ms_p, prof_p, res_p)
+        print("bench: %s %s" % (tag, json.dumps(sweep[tag])),
+              file=sys.stderr, flush=True)
 
-    def step():
-        assert lib.plasma_desc_triangular_create(
-            pa.PlasmaRealDouble, pa.PlasmaLower, nb, nb, n, n, 0, 0, n, n,
-            ctypes.byref(A)) == 0
-        assert lib.plasma_desc_general_create(
-            pa.PlasmaRealDouble, nb, nb, n, nrhs, 0, 0, n, nrhs,
-            ctypes.byref(B)) == 0
-        seq, req = pa.plasma_sequence_t(), pa.plasma_request_t()
-        lib.plasma_sequence_init(ctypes.byref(seq))
-        lib.plasma_request_init(ctypes.byref(req))
-        s, r = ctypes.byref(seq), ctypes.byref(req)
-        lib.plasma_omp_dtr2desc(A0.data_ptr(), n, A, s, r)
-        lib.plasma_omp_dge2desc(B0.data_ptr(), n, B, s, r)
-        lib.plasma_omp_dposv(pa.PlasmaLower, A, B, s, r)
-        lib.plasma_omp_ddesc2tr(A, Aout.data_ptr(), n, s, r)
-        lib.plasma_omp_ddesc2ge(B, X.data_ptr(), n, s, r)
-        status = lib.plasma_sequence_sync(s)
-        lib.plasma_desc_destroy(ctypes.byref(A))
-        lib.plasma_desc_destroy(ctypes.byref(B))
-        if status != 0:
-            raise SystemExit("plasma dposv failed: %d" % status)
+    ggt = None
+    if not args.no_ggt:
+        A0 = ggt_spd(n, seed=0)
+        ms_g, prof_g, res_g = bench.plain(A0, B0, "plain G G^T/n + I")
+        del A0
+        ggt = {"workload": "plain FP64 dposv (drop-in) n=%d nrhs=%d lower, "
+                           "A = G G^T/n + I" % (n, nrhs),
+               "tflops": round(fl / (ms_g * 1e-3) / 1e12, 3),
+               "ms": round(ms_g, 3), "residual_eps": round(res_g, 4),
+               "roofline": roofline(
+                   prof_g, "trailing_update_kernel<double> (FP64 MFMA "
+                   "syrk+gemm)", FP64_MFMA_PEAK),
+               "kernels": classes(prof_g, args.steps)}
 
-    for _ in range(args.warmup):
-        step()
-    print("bench: warmup done (n=%d nb=%d)" % (n, nb), file=sys.stderr,
-          flush=True)
-    lib.plasma_profile_enable(1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step()
-        print("bench: step %d done" % i, file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / args.steps
-
-    def prof(cls):
-        a, f, b, c = (ctypes.c_double(), ctypes.c_double(), ctypes.c_double(),
-                      ctypes.c_long())
-        lib.plasma_profile_query(cls, ctypes.byref(a), ctypes.byref(f),
-                                 ctypes.byref(b), ctypes.byref(c))
-        return a.value, f.value, b.value, c.value
-
-    upd_ms, upd_flops, _, upd_launches = prof(0)
-    classes = {}
-    for cls, name in enumerate(["trailing_update", "potrf_diag", "trsm_panel",
-                                "sweeps", "layout", "mixed"]):
-        a, f, b, c = prof(cls)
-        if c:
-            classes[name] = {"ms_per_step": round(a / args.steps, 3),
-                             "launches_per_step": c // args.steps,
-                             "tflops": round(f / (a * 1e-3) / 1e12, 2)
-                             if a and f else None,
-                             "gbps": round(b / (a * 1e-3) / 1e9, 1)
-                             if a and b else None}
-    lib.plasma_profile_enable(0)
-
-    # verification of the last solution: scaled residual (FP64, on the GPU)
-    Xc = X.T
-    Bc = B0.T
-    R = Bc - A0 @ Xc
-    an = A0.abs().sum(1).max().item()
-    xn = Xc.abs().sum(1).max().item()
-    rn = R.abs().sum(1).max().item()
-    eps = 2.0 ** -53
-    residual = rn / (n * an * xn) / eps
-
-    value = flops_posv(n, nrhs) / (ms * 1e-3) / 1e12
-    achieved = upd_flops / (upd_ms * 1e-3) if upd_ms else 0.0
+    kappa, ms_m, prof_m, res_m, iters, ms_p, prof_p, res_p = head
     line = {
         "metric": "dposv n=32768 time-to-FP64 backward error: mixed-prec vs "
                   "plain FP64; TFLOP/s",
-        "value": round(value, 3),
+        "value": round(fl / (ms_m * 1e-3) / 1e12, 3),
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 3),
+        "ms_per_step": round(ms_m, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": "plain FP64 dposv (drop-in) n=%d nrhs=%d "
-                               "lower, A = G G^T/n + I" % (n, nrhs),
+        "config": {"workload": "plasma_dsposv n=%d nrhs=%d lower, A = Q "
+                               "diag(s) Q^T (64 Householder reflectors), "
+                               "cond %.0e; FP32 factor, FP64 refinement"
+                               % (n, nrhs, kappa),
                    "n": n, "nrhs": nrhs, "nb": nb, "uplo": "lower",
-                   "lookahead": args.lookahead, "precision": "fp64"},
-        "residual_eps": round(residual, 4),
-        "backward_error_ok": residual < 10.0,
-        "roofline": {
-            "kernel": "trailing_update_kernel<double> (FP64 MFMA syrk+gemm)",
-            "bound": "mfma",
-            "achieved": round(achieved / 1e12, 3),
-            "peak": FP64_MFMA_PEAK / 1e12,
-            "unit": "TFLOP/s",
-            "frac": round(achieved / FP64_MFMA_PEAK, 4),
-            "avg_launch_ms": round(upd_ms / upd_launches, 4)
-            if upd_launches else None,
-            "traffic": None,
-        },
-        "kernels": classes,
+                   "lookahead": args.lookahead, "kappa": kappa},
+        "iterations": iters,
+        "residual_eps": round(res_m, 4),
+        "backward_error_ok": res_m < 10.0,
+        "plain_fp64": {"tflops": round(fl / (ms_p * 1e-3) / 1e12, 3),
+                       "ms": round(ms_p, 3), "residual_eps": round(res_p, 4),
+                       "roofline": roofline(
+                           prof_p, "trailing_update_kernel<double> (FP64 "
+                           "MFMA syrk+gemm)", FP64_MFMA_PEAK),
+                       "kernels": classes(prof_p, args.steps)},
+        "speedup_vs_plain": round(ms_p / ms_m, 3),
+        "kappa_sweep": sweep,
+        "roofline": roofline(prof_m, "trailing_update_kernel<float> (FP32 "
+                             "MFMA syrk+gemm of the FP32 factor)",
+                             FP32_MFMA_PEAK),
+        "kernels": classes(prof_m, args.steps),
+        "plain_ggt": ggt,
     }
     if args.cpu_n > 0:
         print("bench: cpu baseline n=%d" % args.cpu_n, file=sys.stderr,
               flush=True)
-        line["cpu_baseline"] = cpu_baseline(args.cpu_n, nrhs, 256,
+        line["cpu_baseline"] = cpu_baseline(args.cpu_n, nrhs, 256, kappa,
                                             args.cpu_threads)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
-    lib.plasma_finalize()
+    bench.lib.plasma_finalize()
 
 
 if __name__ == "__main__":

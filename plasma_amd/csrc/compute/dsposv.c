@@ -115,8 +115,9 @@ void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
     int *st = plasma_sequence_dstatus(sequence, request);
     hipStream_t s = ctx->stream;
     const int n = A.n, nrhs = X.n;
-    /* per-call device scratch: row sums | xnorm | rnorm | anorm | flag */
-    double *work = plasma_dev_alloc((size_t)(A.mt * A.mb + 2 * nrhs + 2) *
+    /* per-call device scratch: norm work | xnorm | rnorm | anorm | flag */
+    const size_t lw = plasma_lansy_work_doubles(A);
+    double *work = plasma_dev_alloc((lw + 2 * (size_t)nrhs + 2) *
                                     sizeof(double));
     if (work == NULL) {
         plasma_request_fail(sequence, request, PlasmaErrorOutOfMemory);
@@ -129,7 +130,7 @@ void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
         return;
     }
     void *inv_d = NULL;
-    double *xnorm = work + (size_t)A.mt * A.mb;
+    double *xnorm = work + lw;
     double *rnorm = xnorm + nrhs;
     double *anorm = rnorm + nrhs;
     int *dflag = (int *)(anorm + 1);

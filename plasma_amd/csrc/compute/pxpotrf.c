@@ -13,6 +13,9 @@
  * With lookahead depth >= 1 the update of step k is split: column k+1 is
  * updated first on the main stream, the panel stream then factors column
  * k+1 (potrf + trsm) while the main stream updates the remaining columns.
+ * The panel solve writes into a panel buffer that the updates read; the
+ * copy into A(:, k) rides along with the bulk update of step k, off the
+ * critical path.
  * Events carry the two dependencies the depend clauses expressed:
  *     panel(k+1) after update(k, k+1)           (ev_col)
  *     update(k+1, ...) after panel(k+1)         (ev_panel)
@@ -73,23 +76,25 @@ static int k_potrf(plasma_enum_t uplo, plasma_desc_t A, int k, void *inv,
 }
 
 static int k_panel(plasma_enum_t uplo, plasma_desc_t A, int k, void *inv,
-                   void *scratch, int *st, hipStream_t s)
+                   void *panel, int *st, hipStream_t s)
 {
     int slot;
     plasma_prof_begin(s, PlasmaProfPanel, panel_flops(A, k), 0, &slot);
-    int err = plasma_kern_trsm_panel(uplo, A, k, inv, scratch, st, s);
+    int err = plasma_kern_trsm_panel(uplo, A, k, inv, panel, st, s);
     plasma_prof_end(s, slot);
     return err;
 }
 
 static int k_update(plasma_enum_t uplo, plasma_desc_t A, int k, int n0,
-                    int n1, int *st, hipStream_t s)
+                    int n1, const void *panel, int copy_back, int *st,
+                    hipStream_t s)
 {
     int slot;
     plasma_context_t *ctx = plasma_context_self();
     plasma_prof_begin(s, PlasmaProfUpdate,
                       ctx->profile ? update_flops(A, n0, n1) : 0, 0, &slot);
-    int err = plasma_kern_trailing_update(uplo, A, k, n0, n1, st, s);
+    int err = plasma_kern_trailing_update(uplo, A, k, n0, n1, panel,
+                                          copy_back, st, s);
     plasma_prof_end(s, slot);
     return err;
 }
@@ -104,6 +109,7 @@ void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A, void *inv,
     hipStream_t s0 = ctx->stream, s1 = ctx->panel_stream;
     const int mt = A.mt;
     size_t es = plasma_element_size(A.precision);
+    const int lookahead = ctx->lookahead > 0 && mt >= 3;
 
     void *own_inv = NULL;
     if (inv == NULL) {
@@ -113,24 +119,31 @@ void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A, void *inv,
             return;
         }
     }
-    void *scratch = plasma_dev_alloc((size_t)imax(mt - 1, 1) * A.mb * A.nb *
-                                     es);
-    if (scratch == NULL) {
+    /* panel buffers: the solved panel of step k lives in panel[k % 2] until
+     * the rest of step k's update has copied it into A(:, k) */
+    const size_t pbytes = (size_t)imax(mt - 1, 1) * A.mb * A.nb * es;
+    void *panel[2] = {NULL, NULL};
+    panel[0] = plasma_dev_alloc(pbytes);
+    panel[1] = lookahead ? plasma_dev_alloc(pbytes) : panel[0];
+    if (panel[0] == NULL || panel[1] == NULL) {
+        if (panel[1] != panel[0])
+            plasma_dev_free(panel[1]);
+        plasma_dev_free(panel[0]);
         plasma_dev_free(own_inv);
         plasma_request_fail(sequence, request, PlasmaErrorOutOfMemory);
         return;
     }
 
-    if (ctx->lookahead == 0 || mt < 3) {
+    if (!lookahead) {
         for (int k = 0; k < mt; k++) {
             if (issue(k_potrf(uplo, A, k, inv, st, s0), sequence, request) ||
-                issue(k_panel(uplo, A, k, inv, scratch, st, s0), sequence,
+                issue(k_panel(uplo, A, k, inv, panel[0], st, s0), sequence,
                       request) ||
-                issue(k_update(uplo, A, k, k + 1, mt, st, s0), sequence,
-                      request))
+                issue(k_update(uplo, A, k, k + 1, mt, panel[0], 1, st, s0),
+                      sequence, request))
                 break;
         }
-        plasma_dev_free(scratch);
+        plasma_dev_free(panel[0]);
         plasma_dev_free(own_inv);
         return;
     }
@@ -146,7 +159,8 @@ void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A, void *inv,
                                                  hipEventDisableTiming),
                          "hipEventCreate")) {
         plasma_request_fail(sequence, request, PlasmaErrorInternal);
-        plasma_dev_free(scratch);
+        plasma_dev_free(panel[1]);
+        plasma_dev_free(panel[0]);
         plasma_dev_free(own_inv);
         return;
     }
@@ -157,31 +171,33 @@ void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A, void *inv,
 
     int err = issue(k_potrf(uplo, A, 0, inv, st, s1), sequence, request);
     if (!err)
-        err = issue(k_panel(uplo, A, 0, inv, scratch, st, s1), sequence,
+        err = issue(k_panel(uplo, A, 0, inv, panel[0], st, s1), sequence,
                     request);
     hipEventRecord(ev_panel, s1);
 
-    for (int k = 0; k < mt && !err; k++) {
+    for (int k = 0; k + 1 < mt && !err; k++) {
+        void *pk = panel[k & 1];
         /* main stream: wait for panel k, update column k+1 */
         hipStreamWaitEvent(s0, ev_panel, 0);
-        if (k + 1 < mt) {
-            err = issue(k_update(uplo, A, k, k + 1, k + 2, st, s0), sequence,
+        err = issue(k_update(uplo, A, k, k + 1, k + 2, pk, 0, st, s0),
+                    sequence, request);
+        hipEventRecord(ev_col, s0);
+        /* panel stream: factor column k+1 into the other panel buffer,
+         * whose last reader (step k-1's update and copy-back) precedes
+         * ev_col on the main stream */
+        hipStreamWaitEvent(s1, ev_col, 0);
+        if (!err)
+            err = issue(k_potrf(uplo, A, k + 1, inv, st, s1), sequence,
                         request);
-            hipEventRecord(ev_col, s0);
-            /* panel stream: factor column k+1 */
-            hipStreamWaitEvent(s1, ev_col, 0);
-            if (!err)
-                err = issue(k_potrf(uplo, A, k + 1, inv, st, s1), sequence,
-                            request);
-            if (!err)
-                err = issue(k_panel(uplo, A, k + 1, inv, scratch, st, s1),
-                            sequence, request);
-            hipEventRecord(ev_panel, s1);
-            /* main stream: the rest of the trailing matrix */
-            if (!err && k + 2 < mt)
-                err = issue(k_update(uplo, A, k, k + 2, mt, st, s0),
-                            sequence, request);
-        }
+        if (!err)
+            err = issue(k_panel(uplo, A, k + 1, inv, panel[(k + 1) & 1], st,
+                                s1), sequence, request);
+        hipEventRecord(ev_panel, s1);
+        /* main stream: the rest of the trailing matrix, and panel k back
+         * into A(:, k) */
+        if (!err)
+            err = issue(k_update(uplo, A, k, k + 2, mt, pk, 1, st, s0),
+                        sequence, request);
     }
     /* join */
     hipEventRecord(ev_panel, s1);
@@ -189,7 +205,8 @@ void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A, void *inv,
     hipEventDestroy(ev_fork);
     hipEventDestroy(ev_col);
     hipEventDestroy(ev_panel);
-    plasma_dev_free(scratch);
+    plasma_dev_free(panel[1]);
+    plasma_dev_free(panel[0]);
     plasma_dev_free(own_inv);
 }
 

@@ -13,8 +13,8 @@
 //    lower_view() returns tile (m, n) of the lower-equivalent factor
 //    L = (stored triangle) for PlasmaLower, or U^T for PlasmaUpper, so that
 //    every algorithm below is written once, for L.
-//  * gemm_nt_accumulate: acc += X * Y^T for a BM x BN block, K streamed in
-//    chunks of KC through double-buffered LDS; 256 threads = 4 waves.
+//  * GemmNT: acc += X * Y^T for a BM x BN block, K streamed in chunks of
+//    KC through double-buffered LDS; 256 threads = 4 waves.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -147,53 +147,106 @@ struct GemmNT {
         T y[2][KC][BN];
     };
 
-    // Coalesced chunk load into registers.  For unit row stride the
-    // consecutive threads walk rows; for unit k stride they walk k.
+    // Chunk loads into registers, W elements per 16-byte vector.  With
+    // `rowmap` (unit row stride, or no unit stride at all) consecutive
+    // threads walk rows: vector v covers rows r..r+W-1 of chunk column kk;
+    // otherwise (unit k stride) they walk k: vector v covers k values
+    // kk..kk+W-1 of row r.  Full, 16-byte aligned chunks with a unit stride
+    // use vector loads; the rest is read element by element with bounds.
+    static constexpr int W = 16 / (int)sizeof(T);
+    static __device__ __forceinline__ bool rowmap(const View<T> &V)
+    {
+        return V.rs == 1 || V.cs != 1;
+    }
+
     template <int ROWS, int L>
     static __device__ __forceinline__ void load(T (&reg)[L], const View<T> &V,
                                                 int nrows, int k0, int K)
     {
+        static_assert(L % W == 0, "whole vectors per thread");
+        typedef T vec_t __attribute__((ext_vector_type(W)));
         const int t = threadIdx.x;
-        if (V.rs == 1) {
+        const bool rm = rowmap(V);
+        const int ld = rm ? V.cs : V.rs;
+        const bool fast = nrows >= ROWS && k0 + KC <= K &&
+                          (rm ? V.rs == 1 : true) &&
+                          ((uintptr_t)V.p % 16 == 0) && (ld % W == 0);
+        // one uniform branch around straight-line load sequences
+        if (rm && fast) {
 #pragma unroll
-            for (int i = 0; i < L; i++) {
-                int e = i * NT + t;
-                int kk = e / ROWS, r = e % ROWS;
-                int kg = k0 + kk;
-                reg[i] = (r < nrows && kg < K) ? V.p[r + (ptrdiff_t)kg * V.cs]
-                                               : T(0);
+            for (int i = 0; i < L / W; i++) {
+                const int v = i * NT + t;
+                const int kk = v / (ROWS / W), r = (v % (ROWS / W)) * W;
+                vec_t x = *reinterpret_cast<const vec_t *>(
+                    V.p + r + (ptrdiff_t)(k0 + kk) * V.cs);
+#pragma unroll
+                for (int u = 0; u < W; u++) reg[i * W + u] = x[u];
+            }
+        }
+        else if (rm) {
+#pragma unroll
+            for (int i = 0; i < L / W; i++) {
+                const int v = i * NT + t;
+                const int kk = v / (ROWS / W), r = (v % (ROWS / W)) * W;
+                const int kg = k0 + kk;
+#pragma unroll
+                for (int u = 0; u < W; u++)
+                    reg[i * W + u] = (r + u < nrows && kg < K)
+                        ? V.p[(ptrdiff_t)(r + u) * V.rs + (ptrdiff_t)kg * V.cs]
+                        : T(0);
+            }
+        }
+        else if (fast) {
+#pragma unroll
+            for (int i = 0; i < L / W; i++) {
+                const int v = i * NT + t;
+                const int r = v / (KC / W), kk = (v % (KC / W)) * W;
+                vec_t x = *reinterpret_cast<const vec_t *>(
+                    V.p + (ptrdiff_t)r * V.rs + (k0 + kk));
+#pragma unroll
+                for (int u = 0; u < W; u++) reg[i * W + u] = x[u];
             }
         }
         else {
 #pragma unroll
-            for (int i = 0; i < L; i++) {
-                int e = i * NT + t;
-                int kk = e % KC, r = e / KC;
-                int kg = k0 + kk;
-                reg[i] = (r < nrows && kg < K)
-                             ? V.p[(ptrdiff_t)r * V.rs + (ptrdiff_t)kg * V.cs]
-                             : T(0);
+            for (int i = 0; i < L / W; i++) {
+                const int v = i * NT + t;
+                const int r = v / (KC / W), kk = (v % (KC / W)) * W;
+                const int kg = k0 + kk;
+#pragma unroll
+                for (int u = 0; u < W; u++)
+                    reg[i * W + u] = (r < nrows && kg + u < K)
+                        ? V.p[(ptrdiff_t)r * V.rs + (kg + u)]
+                        : T(0);
             }
         }
     }
 
+    // Registers -> LDS chunk s[kk][row], with the mapping load() used.
     template <int ROWS, int L>
     static __device__ __forceinline__ void store(const T (&reg)[L],
-                                                 T (*s)[ROWS], bool rowunit)
+                                                 T (*s)[ROWS], bool rm)
     {
         const int t = threadIdx.x;
-        if (rowunit) {
+        if (rm) {
+            typedef T vec_t __attribute__((ext_vector_type(W)));
 #pragma unroll
-            for (int i = 0; i < L; i++) {
-                int e = i * NT + t;
-                s[e / ROWS][e % ROWS] = reg[i];
+            for (int i = 0; i < L / W; i++) {
+                const int v = i * NT + t;
+                const int kk = v / (ROWS / W), r = (v % (ROWS / W)) * W;
+                vec_t x;
+#pragma unroll
+                for (int u = 0; u < W; u++) x[u] = reg[i * W + u];
+                *reinterpret_cast<vec_t *>(&s[kk][r]) = x;
             }
         }
         else {
 #pragma unroll
-            for (int i = 0; i < L; i++) {
-                int e = i * NT + t;
-                s[e % KC][e / KC] = reg[i];
+            for (int i = 0; i < L / W; i++) {
+                const int v = i * NT + t;
+                const int r = v / (KC / W), kk = (v % (KC / W)) * W;
+#pragma unroll
+                for (int u = 0; u < W; u++) s[kk + u][r] = reg[i * W + u];
             }
         }
     }
@@ -216,7 +269,7 @@ struct GemmNT {
         const int wave = threadIdx.x >> 6;
         const int wr = wave / (BN / WN), wc = wave % (BN / WN);
         const int nk = (K + KC - 1) / KC;
-        const bool xu = (X.rs == 1), yu = (Y.rs == 1);
+        const bool xu = rowmap(X), yu = rowmap(Y);
         T rx[LX], ry[LY];
 
         load<BM>(rx, X, xm, 0, K);

@@ -13,18 +13,42 @@ namespace {
 
 constexpr int NTHREADS = 256;
 
-__global__ void fill_double_kernel(double *p, double v, int n)
+// Sum of |V(r, c)| over c < cols for each r < rows into out[r].  With a unit
+// row stride one thread walks each row (consecutive threads read consecutive
+// rows); otherwise one wave walks each row with its lanes along the row and
+// reduces across lanes.  Coalesced either way.
+template <typename T>
+__device__ __forceinline__ void abs_row_sums(const View<T> &V, int rows,
+                                             int cols, double *out)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (V.rs == 1) {
+        for (int r = t; r < rows; r += NTHREADS) {
+            double s = 0;
+            for (int c = 0; c < cols; c++) s += fabs((double)V.at(r, c));
+            out[r] = s;
+        }
+    }
+    else {
+        for (int r = wave; r < rows; r += NTHREADS / 64) {
+            double s = 0;
+            for (int c = lane; c < cols; c += 64) s += fabs((double)V.at(r, c));
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+            if (lane == 0) out[r] = s;
+        }
+    }
 }
 
-// One workgroup per stored tile (m, n), m >= n of the lower-equivalent
-// factor storage: |A| row sums of tile rows go to work[rows of m]; for an
-// off-diagonal tile its column sums are the row sums of the mirrored tile
-// (rows of n).  Diagonal tiles are read symmetrically from their lower part.
+// One workgroup per stored tile (m, n), m >= n, of the lower-equivalent
+// storage.  work is an mt x (mt * mb) array of partial row sums of |A|:
+// slot n of the rows of m holds the row sums of L(m,n), and for m > n slot m
+// of the rows of n holds its column sums (the row sums of the mirrored tile
+// L(m,n)^T).  Every (slot, row) is written by exactly one tile, so no
+// atomics are needed and the result does not depend on scheduling.
+// Diagonal tiles are read symmetrically from their stored triangle.
 template <typename T>
-__global__ __launch_bounds__(NTHREADS) void lansy_rowsum_kernel(
+__global__ __launch_bounds__(NTHREADS) void lansy_tile_kernel(
     plasma_desc_t A, int upper, double *work, const int *status)
 {
     if (block_should_stop(status)) return;
@@ -32,26 +56,50 @@ __global__ __launch_bounds__(NTHREADS) void lansy_rowsum_kernel(
     trapezoid_decode(blockIdx.x, A.mt, 0, &m, &n);
     const int mv = plasma_tile_mview(A, m), nv = plasma_tile_mview(A, n);
     const View<T> L = lower_view<T>(A, upper != 0, m, n);
-    const int t = threadIdx.x;
-    // row sums (one thread per row)
-    for (int r = t; r < mv; r += NTHREADS) {
-        double s = 0;
-        for (int c = 0; c < nv; c++) {
-            if (m == n && c > r) s += fabs((double)L.at(c, r));
-            else                 s += fabs((double)L.at(r, c));
-        }
-        atomicAdd(&work[m * A.mb + r], s);
-    }
-    if (m != n) {
-        for (int c = t; c < nv; c += NTHREADS) {
+    const size_t ld = (size_t)A.mt * A.mb;
+    if (m == n) {
+        for (int r = threadIdx.x; r < mv; r += NTHREADS) {
             double s = 0;
-            for (int r = 0; r < mv; r++) s += fabs((double)L.at(r, c));
-            atomicAdd(&work[n * A.mb + c], s);
+            for (int c = 0; c < nv; c++)
+                s += fabs((double)(c <= r ? L.at(r, c) : L.at(c, r)));
+            work[n * ld + (size_t)m * A.mb + r] = s;
         }
+        return;
     }
+    abs_row_sums<T>(L, mv, nv, work + n * ld + (size_t)m * A.mb);
+    View<T> Lt;
+    Lt.p = L.p;
+    Lt.rs = L.cs;
+    Lt.cs = L.rs;
+    abs_row_sums<T>(Lt, nv, mv, work + m * ld + (size_t)n * A.mb);
 }
 
-// value = max_i work[i]; a NaN row sum propagates (LAPACK's disnan rule)
+// partial[b] = max over the rows i of block b of sum_slot work[slot][i];
+// a NaN row sum propagates (LAPACK's disnan rule).
+__global__ __launch_bounds__(NTHREADS) void lansy_reduce_kernel(
+    const double *work, int mt, size_t ld, int n, double *partial,
+    const int *status)
+{
+    if (block_should_stop(status)) return;
+    __shared__ double red[NTHREADS];
+    const int i = blockIdx.x * NTHREADS + threadIdx.x;
+    double v = 0;
+    if (i < n) {
+        for (int slot = 0; slot < mt; slot++) v += work[slot * ld + i];
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = NTHREADS / 2; s > 0; s /= 2) {
+        if (threadIdx.x < s) {
+            double o = red[threadIdx.x + s];
+            if (o > red[threadIdx.x] || o != o) red[threadIdx.x] = o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// value = max_i work[i]; a NaN propagates (LAPACK's disnan rule)
 __global__ __launch_bounds__(NTHREADS) void max_kernel(const double *work,
                                                        int n, double *value,
                                                        const int *status)
@@ -106,9 +154,13 @@ __global__ __launch_bounds__(NTHREADS) void amax_cols_kernel(
 
 // ---------------------------------------------------------------------------
 // R(m, tn) -= sum_k Asym(m, k) X(k, tn) with Asym(m,k) = L(m,k) (k < m),
-// L(k,m)^T (k > m), and the diagonal tile read symmetrically from its lower
-// part.  64 x 64 output blocks, K streamed over every tile column through
-// double-buffered LDS, MFMA accumulation in the precision of A.
+// L(k,m)^T (k > m), and the diagonal tile read symmetrically from its
+// stored triangle.  64 x 64 output blocks; K runs over every tile column in
+// 16-deep chunks through double-buffered LDS with 16-byte loads (32 KB of
+// LDS, so two workgroups share a CU and hide each other's loads); MFMA
+// accumulation in the precision of A.  The product is formed transposed
+// (X^T Asym^T) so the accumulator's consecutive lanes are consecutive rows
+// of R.
 // ---------------------------------------------------------------------------
 constexpr int SY_BM = 64, SY_BN = 64, SY_KC = 16;
 
@@ -117,7 +169,8 @@ __global__ __launch_bounds__(NTHREADS) void symm_residual_kernel(
     plasma_desc_t A, plasma_desc_t X, plasma_desc_t R, int upper,
     int rchunks, int cchunks, const int *status)
 {
-    typedef GemmNT<T, SY_BM, SY_BN, SY_KC, 32, 32> G;
+    // first operand: 64 right-hand sides; second: 64 rows of Asym
+    typedef GemmNT<T, SY_BN, SY_BM, SY_KC, 32, 32> G;
     __shared__ typename G::Smem sm;
     if (block_should_stop(status)) return;
     const int m = blockIdx.x / rchunks;
@@ -131,56 +184,62 @@ __global__ __launch_bounds__(NTHREADS) void symm_residual_kernel(
     const int t = threadIdx.x;
     const int cpt = (A.mb + SY_KC - 1) / SY_KC;   // chunks per tile column
     const int nchunks = A.mt * cpt;
+    constexpr int W = G::W;
 
     typename G::acc_t acc[G::FM][G::FN];
     G::zero(acc);
-    T ra[G::LX], rb[G::LY];
-    bool xu = true, yu = false;
+    T rp[G::LX], rq[G::LY];
+    bool pu = false, qu = true;
 
-    // Operand views of chunk c: X(row, kk) = Asym(m, kt)(r0 + row, k0 + kk),
-    // Y(col, kk) = X(kt, tn)(k0 + kk, c0 + col).
+    // Chunk c: P(col, kk) = X(kt, tn)(k0 + kk, c0 + col),
+    //          Q(row, kk) = Asym(m, kt)(r0 + row, k0 + kk).
     auto load = [&](int c) {
         const int kt = c / cpt, k0 = (c % cpt) * SY_KC;
         const int K = plasma_tile_mview(A, kt);
-        View<T> V;
-        if (kt < m) {
-            V = lower_view<T>(A, upper != 0, m, kt);
-        }
-        else {
-            const View<T> W = lower_view<T>(A, upper != 0, kt, m);
-            V.p = W.p; V.rs = W.cs; V.cs = W.rs;  // transposed
-        }
-        if (kt != m) {
-            View<T> Vr = V;
-            Vr.p += (ptrdiff_t)r0 * V.rs;
-            G::template load<SY_BM>(ra, Vr, mv - r0, k0, K);
-            xu = (Vr.rs == 1);
-        }
-        else {
-            // diagonal tile: symmetric read of the stored lower triangle
-#pragma unroll
-            for (int i = 0; i < G::LX; i++) {
-                int e = i * NTHREADS + t;
-                int kk = e % SY_KC, r = e / SY_KC;
-                int kg = k0 + kk, row = r0 + r;
-                T v = T(0);
-                if (row < mv && kg < K)
-                    v = row >= kg ? V.at(row, kg) : V.at(kg, row);
-                ra[i] = v;
-            }
-            xu = false;
-        }
         const View<T> Xk = tile_view<T>(X, kt, tn);
-        View<T> Y;
-        Y.p = Xk.p + (ptrdiff_t)c0 * Xk.cs;
-        Y.rs = Xk.cs;
-        Y.cs = 1;
-        G::template load<SY_BN>(rb, Y, nv - c0, k0, K);
-        yu = (Y.rs == 1);
+        View<T> P;
+        P.p = Xk.p + (ptrdiff_t)c0 * Xk.cs;
+        P.rs = Xk.cs;
+        P.cs = 1;
+        G::template load<SY_BN>(rp, P, nv - c0, k0, K);
+        pu = G::rowmap(P);
+        if (kt != m) {
+            View<T> Q;
+            if (kt < m) {
+                Q = lower_view<T>(A, upper != 0, m, kt);
+            }
+            else {
+                const View<T> V = lower_view<T>(A, upper != 0, kt, m);
+                Q.p = V.p; Q.rs = V.cs; Q.cs = V.rs;  // transposed
+            }
+            Q.p += (ptrdiff_t)r0 * Q.rs;
+            G::template load<SY_BM>(rq, Q, mv - r0, k0, K);
+            qu = G::rowmap(Q);
+        }
+        else {
+            // diagonal tile: symmetric read of the stored triangle, with
+            // the row mapping of GemmNT::load
+            const View<T> V = lower_view<T>(A, upper != 0, m, m);
+#pragma unroll
+            for (int i = 0; i < G::LY / W; i++) {
+                const int v = i * NTHREADS + t;
+                const int kk = v / (SY_BM / W), r = (v % (SY_BM / W)) * W;
+                const int kg = k0 + kk;
+#pragma unroll
+                for (int u = 0; u < W; u++) {
+                    const int row = r0 + r + u;
+                    T x = T(0);
+                    if (row < mv && kg < K)
+                        x = row >= kg ? V.at(row, kg) : V.at(kg, row);
+                    rq[i * W + u] = x;
+                }
+            }
+            qu = true;
+        }
     };
     auto store = [&](int buf) {
-        G::template store<SY_BM>(ra, sm.x[buf], xu);
-        G::template store<SY_BN>(rb, sm.y[buf], yu);
+        G::template store<SY_BN>(rp, sm.x[buf], pu);
+        G::template store<SY_BM>(rq, sm.y[buf], qu);
     };
 
     load(0);
@@ -213,10 +272,10 @@ __global__ __launch_bounds__(NTHREADS) void symm_residual_kernel(
     for (int i = 0; i < G::FM; i++)
 #pragma unroll
         for (int j = 0; j < G::FN; j++) {
-            const int cc = c0 + G::acc_col(j);
+            const int rr = r0 + G::acc_col(j);
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int rr = r0 + G::acc_row(i, q);
+                const int cc = c0 + G::acc_row(i, q);
                 if (rr < mv && cc < nv) C.at(rr, cc) -= acc[i][j][q];
             }
         }
@@ -251,26 +310,34 @@ int launch_symm(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t X,
 
 }  // namespace
 
+extern "C" size_t plasma_lansy_work_doubles(plasma_desc_t A)
+{
+    const size_t ld = (size_t)A.mt * A.mb;
+    return (size_t)A.mt * ld + (ld + NTHREADS - 1) / NTHREADS;
+}
+
 extern "C" int plasma_kern_lansy_inf(plasma_enum_t uplo, plasma_desc_t A,
                                      double *work, double *value,
                                      const int *status, hipStream_t s)
 {
-    const int n = A.mt * A.mb;
-    hipLaunchKernelGGL(fill_double_kernel, dim3((n + 255) / 256), dim3(256),
-                       0, s, work, 0.0, n);
+    const size_t ld = (size_t)A.mt * A.mb;
+    const int nblk = (int)((ld + NTHREADS - 1) / NTHREADS);
+    double *partial = work + (size_t)A.mt * ld;
     const long ntiles = (long)A.mt * (A.mt + 1) / 2;
     if (A.precision == PlasmaRealDouble)
-        hipLaunchKernelGGL(lansy_rowsum_kernel<double>, dim3((unsigned)ntiles),
+        hipLaunchKernelGGL(lansy_tile_kernel<double>, dim3((unsigned)ntiles),
                            dim3(NTHREADS), 0, s, A, uplo == PlasmaUpper, work,
                            status);
     else if (A.precision == PlasmaRealFloat)
-        hipLaunchKernelGGL(lansy_rowsum_kernel<float>, dim3((unsigned)ntiles),
+        hipLaunchKernelGGL(lansy_tile_kernel<float>, dim3((unsigned)ntiles),
                            dim3(NTHREADS), 0, s, A, uplo == PlasmaUpper, work,
                            status);
     else
         return PlasmaErrorNotSupported;
-    hipLaunchKernelGGL(max_kernel, dim3(1), dim3(NTHREADS), 0, s, work, n,
-                       value, status);
+    hipLaunchKernelGGL(lansy_reduce_kernel, dim3(nblk), dim3(NTHREADS), 0, s,
+                       work, A.mt, ld, A.m, partial, status);
+    hipLaunchKernelGGL(max_kernel, dim3(1), dim3(NTHREADS), 0, s, partial,
+                       nblk, value, status);
     return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
 }
 

@@ -53,18 +53,24 @@ int plasma_kern_potrf_diag(plasma_enum_t uplo, plasma_desc_t A, int k,
 int plasma_kern_trtri_diag(plasma_enum_t uplo, plasma_desc_t A, void *inv,
                            const int *status, hipStream_t s);
 
-/* Panel solve A(m,k) <- A(m,k) L(k,k)^{-T} = A(m,k) W_kk^T for m > k
- * (reference pzpotrf.c:47-57; upper: :90-99), as an MFMA product into
- * `scratch` ((mt-1) * mb * nb elements) followed by a copy back. */
+/* Panel solve P(m) = A(m,k) L(k,k)^{-T} = A(m,k) W_kk^T for m > k
+ * (reference pzpotrf.c:47-57; upper: :90-99), as an MFMA product into the
+ * panel buffer `panel` ((mt-1) * mb * nb elements; tile m at
+ * panel + (m-k-1) * mb * nb, column major, ld = mb).  A(m,k) itself is
+ * written later, by the copy-back of plasma_kern_trailing_update. */
 int plasma_kern_trsm_panel(plasma_enum_t uplo, plasma_desc_t A, int k,
-                           const void *inv, void *scratch, const int *status,
+                           const void *inv, void *panel, const int *status,
                            hipStream_t s);
 
-/* Trailing update A(m,n) -= A(m,k) A(n,k)^T for n in [n_first, n_end) and
- * m in [n, mt), lower triangle only on diagonal tiles
- * (reference pzpotrf.c:58-78, syrk + gemm tasks; upper: :100-125). */
+/* Trailing update A(m,n) -= P(m) P(n)^T for n in [n_first, n_end) and
+ * m in [n, mt), lower triangle only on diagonal tiles, with the panel P of
+ * step k read from the panel buffer (reference pzpotrf.c:58-78, syrk + gemm
+ * tasks; upper: :100-125).  With copy_back the same launch also stores the
+ * panel into A(m,k), m > k; an empty column range with copy_back is a copy
+ * only. */
 int plasma_kern_trailing_update(plasma_enum_t uplo, plasma_desc_t A, int k,
-                                int n_first, int n_end, const int *status,
+                                int n_first, int n_end, const void *panel,
+                                int copy_back, const int *status,
                                 hipStream_t s);
 
 /*
@@ -98,8 +104,10 @@ int plasma_kern_lacpy(plasma_desc_t A, plasma_desc_t B, const int *status,
 int plasma_kern_geadd(double alpha, plasma_desc_t A, double beta,
                       plasma_desc_t B, const int *status, hipStream_t s);
 /* ||A||_inf of a symmetric matrix stored in one triangle of a general
- * descriptor (reference pzlanhe.c:83-131); work: A.n doubles;
+ * descriptor (reference pzlanhe.c:83-131: per-tile row and column sums, then
+ * a reduction); work: plasma_lansy_work_doubles(A) doubles of device memory;
  * result -> *value (device). */
+size_t plasma_lansy_work_doubles(plasma_desc_t A);
 int plasma_kern_lansy_inf(plasma_enum_t uplo, plasma_desc_t A, double *work,
                           double *value, const int *status, hipStream_t s);
 /* values[j] = max_i |A(i,j)| (reference pdzamax.c:36-52) */

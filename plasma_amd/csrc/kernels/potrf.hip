@@ -17,72 +17,131 @@ constexpr int NTHREADS = 256;
 constexpr int JB = 32;          // column block of the diagonal tile
 
 // ---------------------------------------------------------------------------
-// Wave-level 32 x 32 kernels.  Lane r (r < 32) holds row r of a block in
-// d[0..31]; rows and columns beyond jb are padded with the identity.
-// Values of other lanes are fetched with v_readlane (no LDS, no barriers).
+// Wave-level 32 x 32 kernels.  Lane l holds row r = l % 32 of a block in
+// registers d[0..31] (lanes l and l + 32 compute the same row and store
+// the same values, so the code has no divergent branches, which keeps the
+// unrolled register code free of spills); rows and columns beyond jb are
+// padded with the identity.  Values of other rows are broadcast through
+// LDS: one store per row, then same-address reads, which the LDS serves to
+// all lanes at once.  Blocks move between global memory and registers
+// through an LDS copy (column major, stride LDB = JB + 1), so the register
+// code addresses memory with immediate offsets only.  Only the calling wave
+// touches its LDS buffers; the wavefront-scope fences order its LDS
+// accesses across lanes (a wave's LDS operations complete in program
+// order).
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ T readlane(T v, int l);
+constexpr int LDB = JB + 1;
 
-template <>
-__device__ __forceinline__ double readlane<double>(double v, int l)
+__device__ __forceinline__ void wave_sync_lds()
 {
-    int2 u = __builtin_bit_cast(int2, v);
-    u.x = __builtin_amdgcn_readlane(u.x, l);
-    u.y = __builtin_amdgcn_readlane(u.y, l);
-    return __builtin_bit_cast(double, u);
-}
-
-template <>
-__device__ __forceinline__ float readlane<float>(float v, int l)
-{
-    return __builtin_bit_cast(float,
-        __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Right-looking Cholesky of the block held in d; returns 0 or the 1-based
 // column of the first non-positive (or NaN) pivot (LAPACK dpotf2 rule).
+// sb: 2 * JB elements of LDS.
 template <typename T>
-__device__ __forceinline__ int wave_potrf32(T (&d)[JB], int jb)
+__device__ __forceinline__ int wave_potrf32(T (&d)[JB], int jb, T *sb)
 {
-    const int lane = threadIdx.x & 63;
+    const int r = threadIdx.x & 31;
 #pragma unroll
     for (int j = 0; j < JB; j++) {
-        const T djj = readlane(d[j], j);
+        sb[r] = d[j];                            // column j, unscaled
+        wave_sync_lds();
+        const T djj = sb[j];
         if (j < jb && !(djj > T(0))) return j + 1;
         const T ljj = sqrt(djj);
-        if (lane == j) d[j] = ljj;
-        else if (lane > j) d[j] = d[j] / ljj;
+        const T lrj = (r == j) ? ljj : d[j] / ljj;
+        if (r >= j) d[j] = lrj;
+        sb[JB + r] = lrj;                        // column j of L
+        wave_sync_lds();
 #pragma unroll
         for (int c = j + 1; c < JB; c++) {
-            const T lcj = readlane(d[j], c);
-            if (lane >= c) d[c] -= d[j] * lcj;
+            const T lcj = sb[JB + c];
+            if (r >= c) d[c] -= lrj * lcj;
         }
-        // keep this column's scalar reads next to their use
+        wave_sync_lds();
         __builtin_amdgcn_sched_barrier(0);
     }
     return 0;
 }
 
-// Inverse of the lower-triangular block held in d (rows on lanes):
-// lane c returns column c of L^{-1} in inv[0..31].
+// Inverse of the lower-triangular block held in d (rows on lanes): lane l
+// returns column l % 32 of L^{-1} in inv[0..31].  sl: JB x LDB elements of
+// LDS that receive the block row by row.
 template <typename T>
-__device__ __forceinline__ void wave_trtri32(const T (&d)[JB], T (&inv)[JB])
+__device__ __forceinline__ void wave_trtri32(const T (&d)[JB], T (&inv)[JB],
+                                             T *sl)
+{
+    const int cl = threadIdx.x & 31;
+#pragma unroll
+    for (int c = 0; c < JB; c++) sl[cl * LDB + c] = d[c];
+    wave_sync_lds();
+#pragma unroll
+    for (int r = 0; r < JB; r++) {
+        T s = (cl == r) ? T(1) : T(0);
+#pragma unroll
+        for (int l = 0; l < r; l++) s -= sl[r * LDB + l] * inv[l];
+        inv[r] = s / sl[r * LDB + r];
+        // keep each row's LDS reads next to their use
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    wave_sync_lds();
+}
+
+// sD <- the jb x jb diagonal block at (j0, j0) of the tile view L: lower
+// triangle, identity padding; two columns per pass, coalesced, in-bounds
+// addresses with the bounds applied to the values.
+template <typename T>
+__device__ __forceinline__ void wave_load_block(const View<T> &L, T *sD,
+                                                int j0, int jb)
+{
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 31, h = lane >> 5;
+    for (int c0 = 0; c0 < JB; c0 += 8) {
+        T x[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int c = c0 + 2 * u + h;
+            x[u] = L.at(j0 + min(r, jb - 1), j0 + min(c, jb - 1));
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int c = c0 + 2 * u + h;
+            T v;
+            if (r < jb && c < jb) v = (c <= r) ? x[u] : T(0);
+            else                  v = (r == c) ? T(1) : T(0);
+            sD[c * LDB + r] = v;
+        }
+    }
+    wave_sync_lds();
+}
+
+// Registers -> LDS block: row lane % 32 of d into sD (column major).
+template <typename T>
+__device__ __forceinline__ void wave_rows_to_lds(const T (&d)[JB], T *sD)
+{
+    const int r = threadIdx.x & 31;
+#pragma unroll
+    for (int c = 0; c < JB; c++) sD[c * LDB + r] = d[c];
+}
+
+// LDS block -> registers: row `lane` of sD into d.
+template <typename T>
+__device__ __forceinline__ void wave_rows_from_lds(T (&d)[JB], const T *sD)
 {
     const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int r = 0; r < JB; r++) {
-        T s = (lane == r) ? T(1) : T(0);
-#pragma unroll
-        for (int l = 0; l < r; l++) s -= readlane(d[l], r) * inv[l];
-        inv[r] = s / readlane(d[r], r);
-        __builtin_amdgcn_sched_barrier(0);
-    }
+    for (int c = 0; c < JB; c++) d[c] = sD[c * LDB + (lane & 31)];
 }
 
 // One 16 x 16 MFMA block: acc += A(16 x K) B(16 x K)^T with the operands
 // given as Views positioned at their first row; rows beyond the bounds read
-// as zero.  Result element q of the lane is at (Mfma<T>::row(lane, q),
+// as zero.  Every lane loads from a clamped, in-bounds address and the
+// bounds select the value afterwards, so the loads issue back to back
+// instead of each sitting in its own branch.  Result element q of the lane is at (Mfma<T>::row(lane, q),
 // lane % 16).  Operands are fetched 32 k-values at a time before the MFMAs
 // so the loads of a chunk overlap.
 template <typename T>
@@ -98,8 +157,10 @@ __device__ __forceinline__ void wave_mma_nt(typename Mfma<T>::acc_t &acc,
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             const int kk = k + 4 * u + kq;
-            a[u] = (r < arows && kk < K) ? Av.at(r, kk) : T(0);
-            b[u] = (r < brows && kk < K) ? Bv.at(r, kk) : T(0);
+            const T x = Av.at(min(r, arows - 1), min(kk, K - 1));
+            const T y = Bv.at(min(r, brows - 1), min(kk, K - 1));
+            a[u] = (r < arows && kk < K) ? x : T(0);
+            b[u] = (r < brows && kk < K) ? y : T(0);
         }
 #pragma unroll
         for (int u = 0; u < 8; u++) acc = Mfma<T>::mma(a[u], b[u], acc);
@@ -125,8 +186,10 @@ __device__ __forceinline__ void wave_mma_nt22(typename Mfma<T>::acc_t (&acc)[2][
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const int ra = r + 16 * h;
-                a[h][u] = (ra < arows && kk < K) ? Av.at(ra, kk) : T(0);
-                b[h][u] = (ra < brows && kk < K) ? Bv.at(ra, kk) : T(0);
+                const T x = Av.at(min(ra, arows - 1), min(kk, K - 1));
+                const T y = Bv.at(min(ra, brows - 1), min(kk, K - 1));
+                a[h][u] = (ra < arows && kk < K) ? x : T(0);
+                b[h][u] = (ra < brows && kk < K) ? y : T(0);
             }
         }
 #pragma unroll
@@ -144,6 +207,52 @@ __device__ __forceinline__ View<T> sub(View<T> v, int r, int c)
 {
     v.p += (ptrdiff_t)r * v.rs + (ptrdiff_t)c * v.cs;
     return v;
+}
+
+// Factor and invert the jb x jb diagonal block at (j0, j0) of the tile view
+// L (one wave): the block's L goes back to the tile, its inverse to sW
+// (column major, stride LDB) and to the workspace tile Wv.  sD: LDS block
+// scratch; sb: 2 * JB elements of LDS.  Returns 0 or the block-local 1-based
+// pivot column.
+template <typename T>
+__device__ __forceinline__ int diag_block32(const View<T> &L, const View<T> &Wv,
+                                            T *sD, T *sW, T *sb, int j0, int jb)
+{
+    const int lane = threadIdx.x & 63;
+    wave_load_block(L, sD, j0, jb);
+    T d[JB], iv[JB];
+    wave_rows_from_lds(d, sD);
+    const int info = wave_potrf32(d, jb, sb);
+    if (info) return info;
+    wave_trtri32(d, iv, sD);                      // sD: rows of L, row major
+    wave_rows_to_lds(iv, sW);                     // column lane % 32 of W ...
+    wave_rows_to_lds(d, sD);                      // ... and L, column major
+    wave_sync_lds();
+    // sW holds W^T column major, i.e. W row major: W(r, c) = sW[r * LDB + c]
+    const int r = lane & 31, h = lane >> 5;
+    for (int c = h; c < jb; c += 2) {
+        if (r < jb && r >= c) L.at(j0 + r, j0 + c) = sD[c * LDB + r];
+        if (r < jb) Wv.at(j0 + r, j0 + c) = sW[r * LDB + c];
+    }
+    return 0;
+}
+
+// Invert the factored jb x jb diagonal block at (j0, j0) of L into the
+// workspace tile Wv (one wave; sD, sW: LDS block scratch).
+template <typename T>
+__device__ __forceinline__ void inv_block32(const View<T> &L, const View<T> &Wv,
+                                            T *sD, T *sW, int j0, int jb)
+{
+    const int lane = threadIdx.x & 63;
+    wave_load_block(L, sD, j0, jb);
+    T d[JB], iv[JB];
+    wave_rows_from_lds(d, sD);
+    wave_trtri32(d, iv, sD);
+    wave_rows_to_lds(iv, sW);
+    wave_sync_lds();
+    const int r = lane & 31, h = lane >> 5;
+    for (int c = h; c < jb; c += 2)
+        if (r < jb) Wv.at(j0 + r, j0 + c) = sW[r * LDB + c];
 }
 
 // ---------------------------------------------------------------------------
@@ -174,15 +283,17 @@ __global__ __launch_bounds__(NTHREADS) void potrf_diag_kernel(
     typedef typename Mfma<T>::acc_t acc_t;
 
     constexpr int LDP = JB + 1;          // padded row stride in LDS
-    __shared__ T sDinv[JB][LDP];        // inverse of the current block
+    __shared__ T sDinv[JB * LDB];       // inverse of the current block
+    __shared__ T sDblk[JB * LDB];       // the current block, for wave 0
+    __shared__ T sB[2 * JB];            // broadcast buffer of wave 0
     __shared__ int s_info;
     extern __shared__ unsigned char smem_raw[];
     T *sP = reinterpret_cast<T *>(smem_raw);   // panel rows, n x LDP
     if (t == 0) s_info = 0;
 
     View<T> Dv;                          // Dv(c, l) = Dinv(c, l)
-    Dv.p = &sDinv[0][0];
-    Dv.rs = LDP;
+    Dv.p = sDinv;                        // row major, stride LDB
+    Dv.rs = LDB;
     Dv.cs = 1;
     View<T> Pv;                          // Pv(r, l) = panel row r, column l
     Pv.p = sP;
@@ -195,38 +306,29 @@ __global__ __launch_bounds__(NTHREADS) void potrf_diag_kernel(
             const int i0 = j0 + jb, nrows = n - i0;
             // (a) one wave: factor and invert the diagonal block
             if (wave == 0) {
-                T d[JB];
-#pragma unroll
-                for (int c = 0; c < JB; c++) {
-                    T v;
-                    if (lane < jb && c < jb) v = (c <= lane) ? L.at(j0 + lane, j0 + c) : T(0);
-                    else                     v = (lane == c) ? T(1) : T(0);
-                    d[c] = v;
-                }
-                const int info = wave_potrf32(d, jb);
-                if (info) {
-                    if (lane == 0) s_info = j0 + info;
-                }
-                else {
-                    T iv[JB];
-                    wave_trtri32(d, iv);
-                    if (lane < JB) {
-#pragma unroll
-                        for (int c = 0; c < JB; c++) {
-                            if (lane < jb && c <= lane && c < jb)
-                                L.at(j0 + lane, j0 + c) = d[c];
-                            sDinv[c][lane] = iv[c];   // row c of the inverse
-                            if (lane < jb && c < jb)
-                                Wv.at(j0 + c, j0 + lane) = iv[c];
-                        }
-                    }
-                }
+                const int info = diag_block32<T>(L, Wv, sDblk, sDinv, sB,
+                                                 j0, jb);
+                if (info && lane == 0) s_info = j0 + info;
             }
             else {
-                // the other waves stage the panel rows below the block
-                for (int e = t - 64; e < nrows * JB; e += NTHREADS - 64) {
-                    const int r = e % nrows, c = e / nrows;
-                    sP[r * LDP + c] = (c < jb) ? L.at(i0 + r, j0 + c) : T(0);
+                // the other waves stage the panel rows below the block,
+                // four loads in flight per thread
+                constexpr int STEP = NTHREADS - 64;
+                const int total = nrows * JB;
+                for (int e0 = t - 64; e0 < total; e0 += 4 * STEP) {
+                    T x[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int e = min(e0 + u * STEP, total - 1);
+                        const int r = e % nrows, c = e / nrows;
+                        x[u] = L.at(i0 + r, j0 + min(c, jb - 1));
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int e = e0 + u * STEP;
+                        const int r = e % nrows, c = e / nrows;
+                        if (e < total) sP[r * LDP + c] = (c < jb) ? x[u] : T(0);
+                    }
                 }
             }
             __syncthreads();
@@ -269,7 +371,8 @@ __global__ __launch_bounds__(NTHREADS) void potrf_diag_kernel(
                 for (int q = 0; q < 4; q++) {
                     const int r = bi * 16 + Mfma<T>::row(lane, q);
                     const int c = bj * 16 + lane % 16;
-                    cv[q] = (r < nrows && c <= r) ? L.at(i0 + r, i0 + c) : T(0);
+                    const T x = L.at(i0 + min(r, nrows - 1), i0 + min(c, nrows - 1));
+                    cv[q] = (r < nrows && c <= r) ? x : T(0);
                 }
                 acc_t acc = {0, 0, 0, 0};
                 wave_mma_nt<T>(acc, sub(Pv, bi * 16, 0), nrows - bi * 16,
@@ -288,20 +391,9 @@ __global__ __launch_bounds__(NTHREADS) void potrf_diag_kernel(
         // invert-only: the diagonal blocks' inverses first
         for (int J = wave; J < nbb; J += 4) {
             const int j0 = J * JB, jb = min(JB, n - j0);
-            T d[JB], iv[JB];
-#pragma unroll
-            for (int c = 0; c < JB; c++) {
-                T v;
-                if (lane < jb && c < jb) v = (c <= lane) ? L.at(j0 + lane, j0 + c) : T(0);
-                else                     v = (lane == c) ? T(1) : T(0);
-                d[c] = v;
-            }
-            wave_trtri32(d, iv);
-            if (lane < jb) {
-#pragma unroll
-                for (int c = 0; c < JB; c++)
-                    if (c < jb) Wv.at(j0 + c, j0 + lane) = iv[c];
-            }
+            // per-wave block scratch in the (still unused) panel area
+            inv_block32<T>(L, Wv, sP + 2 * wave * JB * LDB,
+                           sP + (2 * wave + 1) * JB * LDB, j0, jb);
         }
     }
 
@@ -374,13 +466,46 @@ __global__ __launch_bounds__(NTHREADS) void potrf_diag_kernel(
 // ---------------------------------------------------------------------------
 
 // Trailing update of step k, tile columns [n_first, ...): tiles (m, n)
-// enumerated column by column, bpt x bpt blocks of 128 per tile.
+// enumerated column by column, bpt x bpt blocks of 128 per tile.  The solved
+// panel is read from the panel buffer (tile m at panel + (m-k-1) * mb * nb,
+// column major, ld = mb), so both operands have unit row stride for either
+// storage triangle.  The first `ncopy` workgroups copy the panel buffer back
+// into A(m, k), m > k, instead (transposed for upper storage).
 template <typename T>
 struct UpdateDecode {
     plasma_desc_t A;
-    int k, n_first, bpt, upper;
+    int k, n_first, bpt, upper, ncopy;
+    const T *panel;
+    __device__ View<T> panel_tile(int m) const
+    {
+        View<T> v;
+        v.p = const_cast<T *>(panel) + (size_t)(m - k - 1) * A.mb * A.nb;
+        v.rs = 1;
+        v.cs = A.mb;
+        return v;
+    }
     __device__ bool operator()(int b, tg::Job<T> &j) const
     {
+        if (b < ncopy) {
+            const int m = k + 1 + b;
+            const int mv = plasma_tile_mview(A, m), nb = plasma_tile_mview(A, k);
+            const View<T> D = lower_view<T>(A, upper != 0, m, k);
+            const View<T> S = panel_tile(m);
+            if (!upper) {
+                const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+                for (int c = wave; c < nb; c += tg::NT / 64)
+                    for (int r = lane; r < mv; r += 64) D.at(r, c) = S.at(r, c);
+            }
+            else {
+                // D.at(r, c) = p[r * ld + c]: walk c fastest
+                for (int e = threadIdx.x; e < mv * nb; e += tg::NT) {
+                    const int r = e / nb, c = e % nb;
+                    D.at(r, c) = S.at(r, c);
+                }
+            }
+            return false;
+        }
+        b -= ncopy;
         const int per_tile = bpt * bpt;
         const int sub_ = b % per_tile;
         const int bi = sub_ / bpt, bj = sub_ % bpt;
@@ -390,26 +515,26 @@ struct UpdateDecode {
         const int mv = plasma_tile_mview(A, m), nv = plasma_tile_mview(A, n);
         const int r0 = bi * tg::BM, c0 = bj * tg::BM;
         if (r0 >= mv || c0 >= nv) return false;
-        j.X = sub(lower_view<T>(A, upper, m, k), r0, 0);
-        j.Y = sub(lower_view<T>(A, upper, n, k), c0, 0);
+        j.X = sub(panel_tile(m), r0, 0);
+        j.Y = sub(panel_tile(n), c0, 0);
         j.C = sub(lower_view<T>(A, upper, m, n), r0, c0);
         j.xrows = mv - r0;
         j.yrows = nv - c0;
-        j.K = A.nb;
+        j.K = plasma_tile_mview(A, k);
         j.lower = (m == n);
         j.diag_off = c0 - r0;
         return true;
     }
 };
 
-// Panel solve of step k: S(m) = A(m,k) W_kk^T for m > k into the scratch
-// panel (tile m at scratch + (m-k-1) * mb * nb, column major, ld = mb).
+// Panel solve of step k: P(m) = A(m,k) W_kk^T for m > k into the panel
+// buffer (tile m at panel + (m-k-1) * mb * nb, column major, ld = mb).
 template <typename T>
 struct PanelDecode {
     plasma_desc_t A;
     int k, bpt, upper;
     const T *inv;
-    T *scratch;
+    T *panel;
     __device__ bool operator()(int b, tg::Job<T> &j) const
     {
         const int per_tile = bpt * bpt;
@@ -424,7 +549,7 @@ struct PanelDecode {
         W.rs = 1;
         W.cs = A.mb;
         View<T> S;
-        S.p = scratch + (size_t)(m - k - 1) * A.mb * A.nb;
+        S.p = panel + (size_t)(m - k - 1) * A.mb * A.nb;
         S.rs = 1;
         S.cs = A.mb;
         j.X = sub(lower_view<T>(A, upper, m, k), r0, 0);
@@ -439,38 +564,17 @@ struct PanelDecode {
     }
 };
 
-// Copy the solved panel back: A(m,k) = S(m), m > k.
-template <typename T>
-__global__ __launch_bounds__(NTHREADS) void panel_copy_kernel(
-    plasma_desc_t A, int upper, int k, const T *scratch, const int *status)
-{
-    if (block_should_stop(status)) return;
-    const int m = k + 1 + blockIdx.x;
-    const int mv = plasma_tile_mview(A, m), nb = plasma_tile_mview(A, k);
-    const View<T> D = lower_view<T>(A, upper != 0, m, k);
-    const T *s = scratch + (size_t)(m - k - 1) * A.mb * A.nb;
-    if (!upper) {
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        for (int c = wave; c < nb; c += NTHREADS / 64)
-            for (int r = lane; r < mv; r += 64)
-                D.at(r, c) = s[r + (size_t)c * A.mb];
-    }
-    else {
-        // D.at(r, c) = p[r * ld + c]: walk c fastest
-        for (int e = threadIdx.x; e < mv * nb; e += NTHREADS) {
-            const int r = e / nb, c = e % nb;
-            D.at(r, c) = s[r + (size_t)c * A.mb];
-        }
-    }
-}
-
 template <typename T>
 int launch_potrf_diag(plasma_enum_t uplo, plasma_desc_t A, int k0, int count,
                       int factor, void *inv, int *status, hipStream_t s)
 {
     if (count <= 0) return 0;
-    // panel rows (mb x 33) in the factor phase; 4 x 32 x 33 in the inverse
-    const size_t lds = (size_t)max(A.mb, 4 * JB) * (JB + 1) * sizeof(T);
+    // panel rows (mb x 33) in the factor phase; 8 blocks of 32 x 33 for the
+    // invert-only form and 4 in the inverse; the static part is sDinv,
+    // sDblk, sB and the flags
+    const size_t lds = (size_t)max(A.mb, 8 * JB) * (JB + 1) * sizeof(T);
+    const size_t lds_static = (2 * JB * (JB + 1) + 2 * JB) * sizeof(T) + 16;
+    if (lds + lds_static > 160 * 1024) return PlasmaErrorNotSupported;
     if (lds > 65536) {
         static int attr_set[2] = {0, 0};
         int &done = attr_set[sizeof(T) == 8];
@@ -480,7 +584,6 @@ int launch_potrf_diag(plasma_enum_t uplo, plasma_desc_t A, int k0, int count,
                                 (int)lds);
             done = 1;
         }
-        if (lds > 150 * 1024) return PlasmaErrorNotSupported;
     }
     hipLaunchKernelGGL(potrf_diag_kernel<T>, dim3(count), dim3(NTHREADS), lds,
                        s, A, uplo == PlasmaUpper, k0, factor, (T *)inv,
@@ -490,14 +593,14 @@ int launch_potrf_diag(plasma_enum_t uplo, plasma_desc_t A, int k0, int count,
 
 template <typename T>
 int launch_trsm_panel(plasma_enum_t uplo, plasma_desc_t A, int k,
-                      const void *inv, void *scratch, const int *status,
+                      const void *inv, void *panel, const int *status,
                       hipStream_t s)
 {
     const int ntiles = A.mt - k - 1;
     if (ntiles <= 0) return 0;
     const int bpt = (A.mb + tg::BM - 1) / tg::BM;
     PanelDecode<T> d{A, k, bpt, uplo == PlasmaUpper, (const T *)inv,
-                     (T *)scratch};
+                     (T *)panel};
     if (uplo == PlasmaUpper)
         hipLaunchKernelGGL((tg::kernel<T, true, false, false, false, true,
                                        PanelDecode<T>>),
@@ -508,28 +611,31 @@ int launch_trsm_panel(plasma_enum_t uplo, plasma_desc_t A, int k,
                                        PanelDecode<T>>),
                            dim3(ntiles * bpt * bpt), dim3(tg::NT), 0, s, d,
                            status);
-    hipLaunchKernelGGL(panel_copy_kernel<T>, dim3(ntiles), dim3(NTHREADS), 0,
-                       s, A, uplo == PlasmaUpper, k, (const T *)scratch,
-                       status);
     return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
 }
 
 template <typename T>
 int launch_trailing_update(plasma_enum_t uplo, plasma_desc_t A, int k,
-                           int n_first, int n_end, const int *status,
-                           hipStream_t s)
+                           int n_first, int n_end, const void *panel,
+                           int copy_back, const int *status, hipStream_t s)
 {
-    if (n_first >= n_end || n_first >= A.mt) return 0;
     n_end = min(n_end, A.mt);
-    const long T0 = A.mt - n_first;
-    const long ncols = n_end - n_first;
-    const long ntiles = ncols * T0 - ncols * (ncols - 1) / 2;
+    long nblocks = 0;
     const int bpt = (A.mb + tg::BM - 1) / tg::BM;
-    const long nblocks = ntiles * bpt * bpt;
+    if (n_first < n_end) {
+        const long T0 = A.mt - n_first;
+        const long ncols = n_end - n_first;
+        const long ntiles = ncols * T0 - ncols * (ncols - 1) / 2;
+        nblocks = ntiles * bpt * bpt;
+    }
+    const int ncopy = copy_back ? max(A.mt - k - 1, 0) : 0;
+    nblocks += ncopy;
+    if (nblocks == 0) return 0;
     if (nblocks > 0x7fffffffL) return PlasmaErrorNotSupported;
-    UpdateDecode<T> d{A, k, n_first, bpt, uplo == PlasmaUpper};
+    UpdateDecode<T> d{A, k, n_first, bpt, uplo == PlasmaUpper, ncopy,
+                      (const T *)panel};
     if (uplo == PlasmaUpper)
-        hipLaunchKernelGGL((tg::kernel<T, false, true, true, false, false,
+        hipLaunchKernelGGL((tg::kernel<T, false, true, true, true, true,
                                        UpdateDecode<T>>),
                            dim3((unsigned)nblocks), dim3(tg::NT), 0, s, d,
                            status);
@@ -568,26 +674,27 @@ extern "C" int plasma_kern_trtri_diag(plasma_enum_t uplo, plasma_desc_t A,
 }
 
 extern "C" int plasma_kern_trsm_panel(plasma_enum_t uplo, plasma_desc_t A,
-                                      int k, const void *inv, void *scratch,
+                                      int k, const void *inv, void *panel,
                                       const int *status, hipStream_t s)
 {
     if (A.precision == PlasmaRealDouble)
-        return launch_trsm_panel<double>(uplo, A, k, inv, scratch, status, s);
+        return launch_trsm_panel<double>(uplo, A, k, inv, panel, status, s);
     if (A.precision == PlasmaRealFloat)
-        return launch_trsm_panel<float>(uplo, A, k, inv, scratch, status, s);
+        return launch_trsm_panel<float>(uplo, A, k, inv, panel, status, s);
     return PlasmaErrorNotSupported;
 }
 
 extern "C" int plasma_kern_trailing_update(plasma_enum_t uplo,
                                            plasma_desc_t A, int k,
                                            int n_first, int n_end,
+                                           const void *panel, int copy_back,
                                            const int *status, hipStream_t s)
 {
     if (A.precision == PlasmaRealDouble)
         return launch_trailing_update<double>(uplo, A, k, n_first, n_end,
-                                              status, s);
+                                              panel, copy_back, status, s);
     if (A.precision == PlasmaRealFloat)
         return launch_trailing_update<float>(uplo, A, k, n_first, n_end,
-                                             status, s);
+                                             panel, copy_back, status, s);
     return PlasmaErrorNotSupported;
 }

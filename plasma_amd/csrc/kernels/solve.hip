@@ -8,9 +8,9 @@
 // (computed with the factorization, or by plasma_kern_trtri_diag) on MFMA
 // into a scratch block row S = W_kk B(k,:) (or W_kk^T B(k,:)), and one
 // launch then updates every block row still to be solved from S and copies
-// S back into B(k,:).
+// S back into B(k,:).  Rows of a block row are independent, so both
+// launches split the work into blocks of 64 rows x 32 right-hand sides.
 #include "common.hip.h"
-#include "tilegemm.hip.h"
 #include "plasma_kernels.h"
 
 namespace {
@@ -24,61 +24,84 @@ __device__ __forceinline__ View<T> sub(View<T> v, int r, int c)
     return v;
 }
 
-// S(:, cols of tile column tn) = op(W_kk) B(k, tn); S column major with
-// ld = mb, global column index tn * nb + c.
+// Both sweep kernels use GemmNT on blocks of 64 rows x 32 right-hand sides
+// with 32-deep K chunks: the per-step kernels are latency-bound, so small
+// blocks (more workgroups, less serial work each) measured fastest.  The
+// product is formed transposed (Y X^T), so the accumulator's 16 consecutive
+// lanes hold 16 consecutive rows of the column-major result.
+constexpr int SW_BM = 64, SW_BN = 32, SW_KC = 32;
+
+template <typename G, typename T>
+__device__ __forceinline__ void store_rows(
+    const typename G::acc_t (&acc)[G::FM][G::FN], const View<T> &C, int r0,
+    int c0, int mv, int nv, bool subtract)
+{
+#pragma unroll
+    for (int i = 0; i < G::FM; i++)
+#pragma unroll
+        for (int j = 0; j < G::FN; j++) {
+            const int rr = r0 + G::acc_col(j);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int cc = c0 + G::acc_row(i, q);
+                if (rr < mv && cc < nv) {
+                    if (subtract) C.at(rr, cc) -= acc[i][j][q];
+                    else          C.at(rr, cc) = acc[i][j][q];
+                }
+            }
+        }
+}
+
+// S(r0.., c0.. of tile column tn) = op(W_kk) B(k, tn), S column major with
+// ld = mb and global column tn * nb + c.  op(W) = W (forward) is lower
+// triangular, so rows r0.. need k < r0 + 64; W^T (backward) needs k >= r0.
 template <typename T>
-struct SweepDiagDecode {
-    plasma_desc_t A, B;
-    int k, backward, rblocks, cblocks;
-    const T *inv;
-    T *scratch;
-    __device__ bool operator()(int b, tg::Job<T> &j) const
-    {
-        const int per_tn = rblocks * cblocks;
-        const int tn = b / per_tn, rem = b % per_tn;
-        const int r0 = (rem / cblocks) * tg::BM, c0 = (rem % cblocks) * tg::BM;
-        const int mv = plasma_tile_mview(B, k), nv = plasma_tile_nview(B, tn);
-        if (r0 >= mv || c0 >= nv) return false;
-        View<T> W;
-        W.p = const_cast<T *>(inv) + (size_t)k * A.mb * A.mb;
-        if (!backward) { W.rs = 1; W.cs = A.mb; }     // X(r, kk) = W(r, kk)
-        else           { W.rs = A.mb; W.cs = 1; }     // X(r, kk) = W(kk, r)
-        const View<T> Bk = tile_view<T>(B, k, tn);
-        View<T> Y;                                    // Y(c, kk) = Bk(kk, c)
-        Y.p = Bk.p;
-        Y.rs = Bk.cs;
-        Y.cs = 1;
-        View<T> S;
-        S.p = scratch + (size_t)tn * B.nb * A.mb;
-        S.rs = 1;
-        S.cs = A.mb;
-        j.X = sub(W, r0, 0);
-        j.Y = sub(Y, c0, 0);
-        j.C = sub(S, r0, c0);
-        j.xrows = mv - r0;
-        j.yrows = nv - c0;
-        j.K = mv;
-        j.lower = false;
-        j.diag_off = 0;
-        return true;
-    }
-};
+__global__ __launch_bounds__(NTHREADS) void sweep_diag_kernel(
+    plasma_desc_t A, plasma_desc_t B, int backward, int k, int cchunks,
+    const T *inv, T *scratch, const int *status)
+{
+    typedef GemmNT<T, SW_BN, SW_BM, SW_KC, 16, 32> G;
+    __shared__ typename G::Smem sm;
+    if (block_should_stop(status)) return;
+    const int r0 = blockIdx.x * SW_BM;
+    const int tn = blockIdx.y / cchunks;
+    const int c0 = (blockIdx.y % cchunks) * SW_BN;
+    const int mv = plasma_tile_mview(B, k), nv = plasma_tile_nview(B, tn);
+    if (r0 >= mv || c0 >= nv) return;
+    const int klo = backward ? r0 : 0;
+    const int khi = backward ? mv : min(r0 + SW_BM, mv);
+    View<T> X;                                  // X(r, kk) = op(W)(r0 + r, klo + kk)
+    const T *w = inv + (size_t)k * A.mb * A.mb;
+    if (!backward) { X.rs = 1; X.cs = A.mb; }
+    else           { X.rs = A.mb; X.cs = 1; }
+    X.p = const_cast<T *>(w) + (ptrdiff_t)r0 * X.rs + (ptrdiff_t)klo * X.cs;
+    const View<T> Bk = tile_view<T>(B, k, tn);
+    View<T> Y;                                  // Y(c, kk) = B(k)(klo + kk, c0 + c)
+    Y.p = Bk.p + klo + (ptrdiff_t)c0 * Bk.cs;
+    Y.rs = Bk.cs;
+    Y.cs = 1;
+    typename G::acc_t acc[G::FM][G::FN];
+    G::zero(acc);
+    G::run(acc, Y, nv - c0, X, mv - r0, khi - klo, sm);
+    View<T> S;
+    S.p = scratch + (size_t)tn * B.nb * A.mb;
+    S.rs = 1;
+    S.cs = A.mb;
+    store_rows<G, T>(acc, S, r0, c0, mv, nv, false);
+}
 
 // ---------------------------------------------------------------------------
-// Sweep update on MFMA, reading the solved block row from S.
+// Sweep update, reading the solved block row from S.
 // Forward: B(m,:) -= L(m,k) S for m > k.  Backward: B(m,:) -= L(k,m)^T S
 // for m < k.  The last row of workgroups copies S into B(k,:).
-// 64 x 64 output blocks.
 // ---------------------------------------------------------------------------
-constexpr int SW_BM = 64, SW_BN = 64;
-
 template <typename T>
 __global__ __launch_bounds__(NTHREADS) void sweep_update_kernel(
     plasma_desc_t A, plasma_desc_t B, int upper, int backward, int k,
     int ntiles, int rchunks, int cchunks, const T *scratch,
     const int *status)
 {
-    typedef GemmNT<T, SW_BM, SW_BN, 16, 32, 32> G;
+    typedef GemmNT<T, SW_BN, SW_BM, SW_KC, 16, 32> G;
     __shared__ typename G::Smem sm;
     if (block_should_stop(status)) return;
 
@@ -126,18 +149,8 @@ __global__ __launch_bounds__(NTHREADS) void sweep_update_kernel(
 
     typename G::acc_t acc[G::FM][G::FN];
     G::zero(acc);
-    G::run(acc, X, mv - r0, Y, nv - c0, K, sm);
-#pragma unroll
-    for (int i = 0; i < G::FM; i++)
-#pragma unroll
-        for (int j = 0; j < G::FN; j++) {
-            const int cc = c0 + G::acc_col(j);
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int rr = r0 + G::acc_row(i, q);
-                if (rr < mv && cc < nv) C.at(rr, cc) -= acc[i][j][q];
-            }
-        }
+    G::run(acc, Y, nv - c0, X, mv - r0, K, sm);
+    store_rows<G, T>(acc, C, r0, c0, mv, nv, true);
 }
 
 template <typename T>
@@ -145,20 +158,11 @@ int launch_sweep_diag(int backward, plasma_desc_t A, plasma_desc_t B, int k,
                       const void *inv, void *scratch, const int *status,
                       hipStream_t s)
 {
-    const int rblocks = (B.mb + tg::BM - 1) / tg::BM;
-    const int cblocks = (B.nb + tg::BM - 1) / tg::BM;
-    SweepDiagDecode<T> d{A, B, k, backward, rblocks, cblocks,
-                         (const T *)inv, (T *)scratch};
-    if (backward)
-        hipLaunchKernelGGL((tg::kernel<T, true, false, false, false, false,
-                                       SweepDiagDecode<T>>),
-                           dim3(B.nt * rblocks * cblocks), dim3(tg::NT), 0, s,
-                           d, status);
-    else
-        hipLaunchKernelGGL((tg::kernel<T, true, false, false, true, false,
-                                       SweepDiagDecode<T>>),
-                           dim3(B.nt * rblocks * cblocks), dim3(tg::NT), 0, s,
-                           d, status);
+    const int rchunks = (B.mb + SW_BM - 1) / SW_BM;
+    const int cchunks = (B.nb + SW_BN - 1) / SW_BN;
+    hipLaunchKernelGGL(sweep_diag_kernel<T>, dim3(rchunks, B.nt * cchunks),
+                       dim3(NTHREADS), 0, s, A, B, backward, k, cchunks,
+                       (const T *)inv, (T *)scratch, status);
     return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
 }
 

@@ -58,6 +58,18 @@ static int read_conv(plasma_context_t *ctx, const int *dflag, const int *st,
     return flag ? 1 : 0;
 }
 
+/* FP32 solve with the factor in As: through the scaled panels when they
+ * could be allocated, else the plain sweeps */
+static void solve_s(plasma_enum_t uplo, plasma_desc_t As, plasma_desc_t Xs,
+                    const void *inv_s, const void *mpan,
+                    plasma_sequence_t *sequence, plasma_request_t *request)
+{
+    if (mpan != NULL)
+        plasma_pxpotrs_m(As, Xs, inv_s, mpan, sequence, request);
+    else
+        plasma_pxpotrs(uplo, As, Xs, inv_s, sequence, request);
+}
+
 void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
                        plasma_desc_t X, plasma_desc_t As, plasma_desc_t Xs,
                        plasma_desc_t R, double *W, double *Rnorm,
@@ -129,7 +141,7 @@ void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
         plasma_request_fail(sequence, request, PlasmaErrorOutOfMemory);
         return;
     }
-    void *inv_d = NULL;
+    void *inv_d = NULL, *mpan = NULL;
     double *xnorm = work + lw;
     double *rnorm = xnorm + nrhs;
     double *anorm = rnorm + nrhs;
@@ -145,7 +157,12 @@ void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
         issue(plasma_kern_lag2(A, As, st, s), sequence, request))
         goto done;
     plasma_pxpotrf(uplo, As, inv_s, sequence, request);
-    plasma_pxpotrs(uplo, As, Xs, inv_s, sequence, request);
+    /* the refinement solves with this factor up to 31 times: scale the
+     * panels once so that every sweep step is one launch */
+    mpan = plasma_dev_alloc(plasma_mpanel_bytes(As));
+    if (mpan != NULL)
+        plasma_pxscale_panels(uplo, As, inv_s, mpan, sequence, request);
+    solve_s(uplo, As, Xs, inv_s, mpan, sequence, request);
     if (issue(plasma_kern_lag2(Xs, X, st, s), sequence, request))
         goto done;
 
@@ -166,7 +183,7 @@ void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
     for (int it = 0; it < itermax; it++) {
         if (issue(plasma_kern_lag2(R, Xs, st, s), sequence, request))
             goto done;
-        plasma_pxpotrs(uplo, As, Xs, inv_s, sequence, request);
+        solve_s(uplo, As, Xs, inv_s, mpan, sequence, request);
         if (issue(plasma_kern_lag2(Xs, R, st, s), sequence, request) ||
             issue(plasma_kern_geadd(1.0, R, 1.0, X, st, s), sequence,
                   request) ||
@@ -203,6 +220,7 @@ void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
     plasma_pxpotrs(uplo, A, X, inv_d, sequence, request);
 
 done:
+    plasma_dev_free(mpan);
     plasma_dev_free(inv_d);
     plasma_dev_free(inv_s);
     plasma_dev_free(work);

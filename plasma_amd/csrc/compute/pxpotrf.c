@@ -288,3 +288,64 @@ void plasma_pxpotrs(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
                        A, B, inv, sequence, request);
     plasma_dev_free(own_inv);
 }
+
+/*
+ * Repeated solves with one factor (the refinement loop of dsposv).  With
+ * L = M D, M(m,k) = L(m,k) W_kk (m > k), M(k,k) = I and D = diag(L_kk):
+ *     L Y = B      <=>  Z = M^{-1} B,  Y = D^{-1} Z
+ *     L^T X = Y    <=>  M^T X = D^{-T} Y = W^T W Z
+ * so each sweep step is one launch (B(k,:) is final when step k runs)
+ * instead of a diagonal solve plus an update; the two diagonal products
+ * run over all block rows at once.  The reference's pztrsm sweeps
+ * (compute/pztrsm.c:113-177) compute the same solution.
+ */
+size_t plasma_mpanel_bytes(plasma_desc_t A)
+{
+    return (size_t)A.mt * (A.mt - 1) / 2 * A.mb * A.nb *
+           plasma_element_size(A.precision);
+}
+
+void plasma_pxscale_panels(plasma_enum_t uplo, plasma_desc_t A,
+                           const void *inv, void *M,
+                           plasma_sequence_t *sequence,
+                           plasma_request_t *request)
+{
+    if (sequence->status != PlasmaSuccess)
+        return;
+    plasma_context_t *ctx = plasma_context_self();
+    int *st = plasma_sequence_dstatus(sequence, request);
+    issue(plasma_kern_scale_panels(uplo, A, inv, M, st, ctx->stream),
+          sequence, request);
+}
+
+void plasma_pxpotrs_m(plasma_desc_t A, plasma_desc_t B, const void *inv,
+                      const void *M, plasma_sequence_t *sequence,
+                      plasma_request_t *request)
+{
+    if (sequence->status != PlasmaSuccess)
+        return;
+    plasma_context_t *ctx = plasma_context_self();
+    int *st = plasma_sequence_dstatus(sequence, request);
+    hipStream_t s0 = ctx->stream;
+    const int mt = B.mt;
+    size_t es = plasma_element_size(B.precision);
+    void *scratch = plasma_dev_alloc((size_t)mt * A.mb * B.nt * B.nb * es);
+    if (scratch == NULL) {
+        plasma_request_fail(sequence, request, PlasmaErrorOutOfMemory);
+        return;
+    }
+    int err = 0, slot;
+    const double sweep_flops = 2.0 * B.m * B.m * B.n;  /* both sweeps */
+    plasma_prof_begin(s0, PlasmaProfSweep, sweep_flops, 0, &slot);
+    for (int k = 0; k + 1 < mt && !err; k++)
+        err = issue(plasma_kern_msweep_update(0, A, B, k, M, st, s0),
+                    sequence, request);
+    if (!err)
+        err = issue(plasma_kern_diag_normal(A, B, inv, scratch, st, s0),
+                    sequence, request);
+    for (int k = mt - 1; k > 0 && !err; k--)
+        err = issue(plasma_kern_msweep_update(1, A, B, k, M, st, s0),
+                    sequence, request);
+    plasma_prof_end(s0, slot);
+    plasma_dev_free(scratch);
+}

@@ -128,6 +128,17 @@ void plasma_pxtrsm_left(plasma_enum_t uplo, plasma_enum_t trans,
                         plasma_request_t *request);
 /* bytes of the inverse workspace for A */
 size_t plasma_inv_bytes(plasma_desc_t A);
+/* Repeated solves with one factor: M(m,k) = L(m,k) W_kk for m > k
+ * (plasma_mpanel_bytes(A) bytes), computed once by pxscale_panels; then
+ * pxpotrs_m solves with one launch per sweep step. */
+size_t plasma_mpanel_bytes(plasma_desc_t A);
+void plasma_pxscale_panels(plasma_enum_t uplo, plasma_desc_t A,
+                           const void *inv, void *M,
+                           plasma_sequence_t *sequence,
+                           plasma_request_t *request);
+void plasma_pxpotrs_m(plasma_desc_t A, plasma_desc_t B, const void *inv,
+                      const void *M, plasma_sequence_t *sequence,
+                      plasma_request_t *request);
 
 /* layout translation between a LAPACK array (host or device) and tiles */
 void plasma_pxlapack2desc(plasma_enum_t region, const void *pA, int lda,

@@ -91,6 +91,24 @@ int plasma_kern_sweep_update(plasma_enum_t uplo, int backward,
                              hipStream_t s);
 
 /*
+ * Sweeps for repeated solves with one factor.  plasma_kern_scale_panels
+ * stores M(m,k) = L(m,k) W_kk for every m > k into M (packed strictly lower
+ * tiles, column by column; (mt-1) mt / 2 tiles of mb * nb elements).  With
+ * L = M diag(L_kk): forward Z = M^{-1} B is plasma_kern_msweep_update(0, k)
+ * for k = 0 .. mt-2; plasma_kern_diag_normal replaces every B(k,:) by
+ * W_kk^T W_kk B(k,:) (scratch: mt * mb * nt * nb elements); the backward
+ * sweep M^T X = V is plasma_kern_msweep_update(1, k) for k = mt-1 .. 1.
+ */
+int plasma_kern_scale_panels(plasma_enum_t uplo, plasma_desc_t A,
+                             const void *inv, void *M, const int *status,
+                             hipStream_t s);
+int plasma_kern_msweep_update(int backward, plasma_desc_t A, plasma_desc_t B,
+                              int k, const void *M, const int *status,
+                              hipStream_t s);
+int plasma_kern_diag_normal(plasma_desc_t A, plasma_desc_t B, const void *inv,
+                            void *scratch, const int *status, hipStream_t s);
+
+/*
  * Mixed-precision helpers (reference compute/zcposv.c:363-539).
  */
 /* FP64 -> FP32 (round to nearest), sets *status = 1 on overflow

@@ -11,6 +11,7 @@
 // S back into B(k,:).  Rows of a block row are independent, so both
 // launches split the work into blocks of 64 rows x 32 right-hand sides.
 #include "common.hip.h"
+#include "tilegemm.hip.h"
 #include "plasma_kernels.h"
 
 namespace {
@@ -55,14 +56,20 @@ __device__ __forceinline__ void store_rows(
 // S(r0.., c0.. of tile column tn) = op(W_kk) B(k, tn), S column major with
 // ld = mb and global column tn * nb + c.  op(W) = W (forward) is lower
 // triangular, so rows r0.. need k < r0 + 64; W^T (backward) needs k >= r0.
+// Tile k = k0 + blockIdx.z; its S slab starts at scratch + blockIdx.z *
+// kstride.  With from_s the right-hand side is read from the slab instead
+// of B(k, :), with to_b the product is stored into B(k, :) instead of the
+// slab (the two are never both set).
 template <typename T>
 __global__ __launch_bounds__(NTHREADS) void sweep_diag_kernel(
-    plasma_desc_t A, plasma_desc_t B, int backward, int k, int cchunks,
-    const T *inv, T *scratch, const int *status)
+    plasma_desc_t A, plasma_desc_t B, int backward, int k0, int cchunks,
+    const T *inv, T *scratch, size_t kstride, int from_s, int to_b,
+    const int *status)
 {
     typedef GemmNT<T, SW_BN, SW_BM, SW_KC, 16, 32> G;
     __shared__ typename G::Smem sm;
     if (block_should_stop(status)) return;
+    const int k = k0 + blockIdx.z;
     const int r0 = blockIdx.x * SW_BM;
     const int tn = blockIdx.y / cchunks;
     const int c0 = (blockIdx.y % cchunks) * SW_BN;
@@ -75,19 +82,20 @@ __global__ __launch_bounds__(NTHREADS) void sweep_diag_kernel(
     if (!backward) { X.rs = 1; X.cs = A.mb; }
     else           { X.rs = A.mb; X.cs = 1; }
     X.p = const_cast<T *>(w) + (ptrdiff_t)r0 * X.rs + (ptrdiff_t)klo * X.cs;
+    View<T> S;
+    S.p = scratch + blockIdx.z * kstride + (size_t)tn * B.nb * A.mb;
+    S.rs = 1;
+    S.cs = A.mb;
     const View<T> Bk = tile_view<T>(B, k, tn);
-    View<T> Y;                                  // Y(c, kk) = B(k)(klo + kk, c0 + c)
-    Y.p = Bk.p + klo + (ptrdiff_t)c0 * Bk.cs;
-    Y.rs = Bk.cs;
+    const View<T> &Src = from_s ? S : Bk;
+    View<T> Y;                                  // Y(c, kk) = Src(klo + kk, c0 + c)
+    Y.p = Src.p + klo + (ptrdiff_t)c0 * Src.cs;
+    Y.rs = Src.cs;
     Y.cs = 1;
     typename G::acc_t acc[G::FM][G::FN];
     G::zero(acc);
     G::run(acc, Y, nv - c0, X, mv - r0, khi - klo, sm);
-    View<T> S;
-    S.p = scratch + (size_t)tn * B.nb * A.mb;
-    S.rs = 1;
-    S.cs = A.mb;
-    store_rows<G, T>(acc, S, r0, c0, mv, nv, false);
+    store_rows<G, T>(acc, to_b ? Bk : S, r0, c0, mv, nv, false);
 }
 
 // ---------------------------------------------------------------------------
@@ -162,7 +170,8 @@ int launch_sweep_diag(int backward, plasma_desc_t A, plasma_desc_t B, int k,
     const int cchunks = (B.nb + SW_BN - 1) / SW_BN;
     hipLaunchKernelGGL(sweep_diag_kernel<T>, dim3(rchunks, B.nt * cchunks),
                        dim3(NTHREADS), 0, s, A, B, backward, k, cchunks,
-                       (const T *)inv, (T *)scratch, status);
+                       (const T *)inv, (T *)scratch, (size_t)0, 0, 0,
+                       status);
     return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
 }
 
@@ -179,6 +188,157 @@ int launch_sweep_update(plasma_enum_t uplo, int backward, plasma_desc_t A,
                        dim3(NTHREADS), 0, s, A, B, uplo == PlasmaUpper,
                        backward, k, ntiles, rchunks, cchunks,
                        (const T *)scratch, status);
+    return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
+}
+
+// ---------------------------------------------------------------------------
+// Sweeps with the scaled panels M(m,k) = L(m,k) W_kk, m > k (all strictly
+// lower tiles, packed column by column: tile (m, k) at
+// M + (k (mt-1) - k (k-1) / 2 + m - k - 1) * mb * nb, column major, ld mb).
+// With L = M D, D = diag(L_kk), the forward sweep is Z = M^{-1} B (unit
+// block diagonal: one update launch per step), Y = D^{-1} Z, and the
+// backward sweep solves M^T X = D^{-T} Y: per step one launch again.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ size_t mpanel_index(int mt, int m, int k)
+{
+    return (size_t)k * (mt - 1) - (size_t)k * (k - 1) / 2 + (m - k - 1);
+}
+
+template <typename T>
+struct ScaleDecode {
+    plasma_desc_t A;
+    int bpt, upper;
+    const T *inv;
+    T *M;
+    __device__ bool operator()(int b, tg::Job<T> &j) const
+    {
+        const int per_tile = bpt * bpt;
+        const int t = b / per_tile, sub_ = b % per_tile;
+        const int bi = sub_ / bpt, bj = sub_ % bpt;
+        int mm, k;
+        trapezoid_decode(t, A.mt - 1, 0, &mm, &k);
+        const int m = mm + 1;
+        const int mv = plasma_tile_mview(A, m), nb = plasma_tile_mview(A, k);
+        const int r0 = bi * tg::BM, c0 = bj * tg::BM;
+        if (r0 >= mv || c0 >= nb) return false;
+        View<T> Y;                              // Y(c, kk) = W_kk(kk, c)
+        Y.p = const_cast<T *>(inv) + (size_t)k * A.mb * A.mb +
+              (size_t)c0 * A.mb;
+        Y.rs = A.mb;
+        Y.cs = 1;
+        View<T> C;
+        C.p = M + (size_t)t * A.mb * A.nb + r0 + (size_t)c0 * A.mb;
+        C.rs = 1;
+        C.cs = A.mb;
+        j.X = sub(lower_view<T>(A, upper, m, k), r0, 0);
+        j.Y = Y;
+        j.C = C;
+        j.xrows = mv - r0;
+        j.yrows = nb - c0;
+        j.K = nb;
+        j.lower = false;
+        j.diag_off = 0;
+        return true;
+    }
+};
+
+template <typename T>
+int launch_scale_panels(plasma_enum_t uplo, plasma_desc_t A, const void *inv,
+                        void *M, const int *status, hipStream_t s)
+{
+    if (A.mt < 2) return 0;
+    const int bpt = (A.mb + tg::BM - 1) / tg::BM;
+    const long nblocks = (long)A.mt * (A.mt - 1) / 2 * bpt * bpt;
+    if (nblocks > 0x7fffffffL) return PlasmaErrorNotSupported;
+    ScaleDecode<T> d{A, bpt, uplo == PlasmaUpper, (const T *)inv, (T *)M};
+    if (uplo == PlasmaUpper)
+        hipLaunchKernelGGL((tg::kernel<T, true, false, false, false, false,
+                                       ScaleDecode<T>>),
+                           dim3((unsigned)nblocks), dim3(tg::NT), 0, s, d,
+                           status);
+    else
+        hipLaunchKernelGGL((tg::kernel<T, true, false, false, true, false,
+                                       ScaleDecode<T>>),
+                           dim3((unsigned)nblocks), dim3(tg::NT), 0, s, d,
+                           status);
+    return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
+}
+
+// Forward: B(m,:) -= M(m,k) B(k,:) for m > k.  Backward: B(j,:) -=
+// M(k,j)^T B(k,:) for j < k.  B(k,:) is final when step k runs.
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void msweep_update_kernel(
+    plasma_desc_t A, plasma_desc_t B, int backward, int k, int rchunks,
+    int cchunks, const T *M, const int *status)
+{
+    typedef GemmNT<T, SW_BN, SW_BM, SW_KC, 16, 32> G;
+    __shared__ typename G::Smem sm;
+    if (block_should_stop(status)) return;
+    const int tn = blockIdx.y / cchunks;
+    const int c0 = (blockIdx.y % cchunks) * SW_BN;
+    const int nv = plasma_tile_nview(B, tn);
+    const int K = plasma_tile_mview(B, k);
+    const int m = backward ? (int)(blockIdx.x / rchunks)
+                           : k + 1 + (int)(blockIdx.x / rchunks);
+    const int r0 = (blockIdx.x % rchunks) * SW_BM;
+    const int mv = plasma_tile_mview(B, m);
+    if (r0 >= mv || c0 >= nv) return;
+    View<T> X;
+    if (!backward) {                            // X(r, kk) = M(m,k)(r, kk)
+        X.p = const_cast<T *>(M) + mpanel_index(A.mt, m, k) * A.mb * A.nb +
+              r0;
+        X.rs = 1;
+        X.cs = A.mb;
+    }
+    else {                                      // X(r, kk) = M(k,m)(kk, r)
+        X.p = const_cast<T *>(M) + mpanel_index(A.mt, k, m) * A.mb * A.nb +
+              (size_t)r0 * A.mb;
+        X.rs = A.mb;
+        X.cs = 1;
+    }
+    const View<T> Bk = tile_view<T>(B, k, tn);
+    View<T> Y;                                   // Y(c, kk) = B(k)(kk, c0 + c)
+    Y.p = Bk.p + (ptrdiff_t)c0 * Bk.cs;
+    Y.rs = Bk.cs;
+    Y.cs = 1;
+    typename G::acc_t acc[G::FM][G::FN];
+    G::zero(acc);
+    G::run(acc, Y, nv - c0, X, mv - r0, K, sm);
+    store_rows<G, T>(acc, tile_view<T>(B, m, tn), r0, c0, mv, nv, true);
+}
+
+template <typename T>
+int launch_msweep_update(int backward, plasma_desc_t A, plasma_desc_t B,
+                         int k, const void *M, const int *status,
+                         hipStream_t s)
+{
+    const int ntiles = backward ? k : B.mt - k - 1;
+    if (ntiles <= 0) return 0;
+    const int rchunks = (B.mb + SW_BM - 1) / SW_BM;
+    const int cchunks = (B.nb + SW_BN - 1) / SW_BN;
+    hipLaunchKernelGGL(msweep_update_kernel<T>,
+                       dim3(ntiles * rchunks, B.nt * cchunks), dim3(NTHREADS),
+                       0, s, A, B, backward, k, rchunks, cchunks,
+                       (const T *)M, status);
+    return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
+}
+
+// B(k,:) = W_kk^T W_kk B(k,:) for every k: two launches over all tiles
+// through the scratch (mt slabs of mb x (nt * nb)).
+template <typename T>
+int launch_diag_both(plasma_desc_t A, plasma_desc_t B, const void *inv,
+                     void *scratch, const int *status, hipStream_t s)
+{
+    const int rchunks = (B.mb + SW_BM - 1) / SW_BM;
+    const int cchunks = (B.nb + SW_BN - 1) / SW_BN;
+    const size_t kstride = (size_t)A.mb * B.nt * B.nb;
+    dim3 grid(rchunks, B.nt * cchunks, B.mt);
+    hipLaunchKernelGGL(sweep_diag_kernel<T>, grid, dim3(NTHREADS), 0, s, A, B,
+                       0, 0, cchunks, (const T *)inv, (T *)scratch, kstride,
+                       0, 0, status);
+    hipLaunchKernelGGL(sweep_diag_kernel<T>, grid, dim3(NTHREADS), 0, s, A, B,
+                       1, 0, cchunks, (const T *)inv, (T *)scratch, kstride,
+                       1, 1, status);
     return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
 }
 
@@ -213,5 +373,41 @@ extern "C" int plasma_kern_sweep_update(plasma_enum_t uplo, int backward,
     if (A.precision == PlasmaRealFloat)
         return launch_sweep_update<float>(uplo, backward, A, B, k, scratch,
                                           status, s);
+    return PlasmaErrorNotSupported;
+}
+
+extern "C" int plasma_kern_scale_panels(plasma_enum_t uplo, plasma_desc_t A,
+                                        const void *inv, void *M,
+                                        const int *status, hipStream_t s)
+{
+    if (A.precision == PlasmaRealDouble)
+        return launch_scale_panels<double>(uplo, A, inv, M, status, s);
+    if (A.precision == PlasmaRealFloat)
+        return launch_scale_panels<float>(uplo, A, inv, M, status, s);
+    return PlasmaErrorNotSupported;
+}
+
+extern "C" int plasma_kern_msweep_update(int backward, plasma_desc_t A,
+                                         plasma_desc_t B, int k,
+                                         const void *M, const int *status,
+                                         hipStream_t s)
+{
+    if (A.precision != B.precision) return PlasmaErrorIllegalValue;
+    if (A.precision == PlasmaRealDouble)
+        return launch_msweep_update<double>(backward, A, B, k, M, status, s);
+    if (A.precision == PlasmaRealFloat)
+        return launch_msweep_update<float>(backward, A, B, k, M, status, s);
+    return PlasmaErrorNotSupported;
+}
+
+extern "C" int plasma_kern_diag_normal(plasma_desc_t A, plasma_desc_t B,
+                                       const void *inv, void *scratch,
+                                       const int *status, hipStream_t s)
+{
+    if (A.precision != B.precision) return PlasmaErrorIllegalValue;
+    if (A.precision == PlasmaRealDouble)
+        return launch_diag_both<double>(A, B, inv, scratch, status, s);
+    if (A.precision == PlasmaRealFloat)
+        return launch_diag_both<float>(A, B, inv, scratch, status, s);
     return PlasmaErrorNotSupported;
 }

@@ -315,6 +315,7 @@ def main():
     kappas = [float(k) for k in args.kappa.split(",") if k]
     g = torch.Generator(device="cuda").manual_seed(1)
     B0 = torch.randn(nrhs, n, dtype=torch.float64, device="cuda", generator=g)
+    torch.cuda.synchronize()
 
     sweep = {}
     head = None
@@ -342,6 +343,7 @@ def main():
     ggt = None
     if not args.no_ggt:
         A0 = ggt_spd(n, seed=0)
+        torch.cuda.synchronize()   # the library's streams read A0 next
         ms_g, prof_g, res_g = bench.plain(A0, B0, "plain G G^T/n + I")
         del A0
         ggt = {"workload": "plain FP64 dposv (drop-in) n=%d nrhs=%d lower, "

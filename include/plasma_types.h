@@ -116,6 +116,7 @@ enum {
     PlasmaLookahead        = 1001, /* depth of the panel lookahead, 0..4 */
     PlasmaGraphCapture     = 1002, /* PlasmaEnabled: replay hipGraphs */
     PlasmaDevice           = 1003, /* HIP device ordinal (read only) */
+    PlasmaUpdateDepth      = 1004, /* tile columns per trailing update, 1..4 */
     PlasmaParamUnknown     = INT_MAX
 };
 

@@ -57,6 +57,7 @@ static void context_defaults(plasma_context_t *ctx)
     ctx->max_panel_threads = 1;
     ctx->householder_mode = PlasmaFlatHouseholder;
     ctx->lookahead = 1;
+    ctx->group = 2;
     ctx->graph_capture = PlasmaDisabled;
 }
 
@@ -185,6 +186,13 @@ int plasma_set(plasma_enum_t param, int value)
         }
         ctx->lookahead = value;
         break;
+    case PlasmaUpdateDepth:
+        if (value < 1 || value > 4) {
+            plasma_error("invalid update depth");
+            return PlasmaErrorIllegalValue;
+        }
+        ctx->group = value;
+        break;
     case PlasmaGraphCapture:
         if (value != PlasmaEnabled && value != PlasmaDisabled) {
             plasma_error("invalid graph capture flag");
@@ -213,6 +221,7 @@ int plasma_get(plasma_enum_t param, int *value)
     case PlasmaLookahead:       *value = ctx->lookahead;         break;
     case PlasmaGraphCapture:    *value = ctx->graph_capture;     break;
     case PlasmaDevice:          *value = ctx->device;            break;
+    case PlasmaUpdateDepth:     *value = ctx->group;             break;
     default:
         plasma_error("Unknown parameter");
         return PlasmaErrorIllegalValue;

@@ -55,6 +55,7 @@ struct plasma_context_s {
 
     /* extensions */
     int lookahead;        /* panel lookahead depth, 0 disables */
+    int group;            /* tile columns whose panels update together */
     int graph_capture;    /* replay cached hipGraphs for the sync drivers */
 
     /* device */

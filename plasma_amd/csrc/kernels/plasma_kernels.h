@@ -53,25 +53,34 @@ int plasma_kern_potrf_diag(plasma_enum_t uplo, plasma_desc_t A, int k,
 int plasma_kern_trtri_diag(plasma_enum_t uplo, plasma_desc_t A, void *inv,
                            const int *status, hipStream_t s);
 
+/* Panel buffers.  The solved panels of a group of g consecutive tile
+ * columns pbase .. pbase+g-1 share one buffer of (mt-1) tiles of
+ * mb x pcols elements (pcols = g * nb): tile m (> pbase) at
+ * panel + (m - pbase - 1) * mb * pcols, column major, ld = mb, whose
+ * columns [j * nb, (j + 1) * nb) hold P(m) of tile column pbase + j. */
+
 /* Panel solve P(m) = A(m,k) L(k,k)^{-T} = A(m,k) W_kk^T for m > k
- * (reference pzpotrf.c:47-57; upper: :90-99), as an MFMA product into the
- * panel buffer `panel` ((mt-1) * mb * nb elements; tile m at
- * panel + (m-k-1) * mb * nb, column major, ld = mb).  A(m,k) itself is
- * written later, by the copy-back of plasma_kern_trailing_update. */
+ * (reference pzpotrf.c:47-57; upper: :90-99), as an MFMA product into
+ * column block pcol of the group buffer.  A(m,k) itself is written later,
+ * by the copy-back of plasma_kern_trailing_update. */
 int plasma_kern_trsm_panel(plasma_enum_t uplo, plasma_desc_t A, int k,
-                           const void *inv, void *panel, const int *status,
-                           hipStream_t s);
+                           const void *inv, void *panel, int pbase, int pcols,
+                           int pcol, const int *status, hipStream_t s);
 
 /* Trailing update A(m,n) -= P(m) P(n)^T for n in [n_first, n_end) and
- * m in [n, mt), lower triangle only on diagonal tiles, with the panel P of
- * step k read from the panel buffer (reference pzpotrf.c:58-78, syrk + gemm
- * tasks; upper: :100-125).  With copy_back the same launch also stores the
- * panel into A(m,k), m > k; an empty column range with copy_back is a copy
- * only. */
-int plasma_kern_trailing_update(plasma_enum_t uplo, plasma_desc_t A, int k,
-                                int n_first, int n_end, const void *panel,
-                                int copy_back, const int *status,
-                                hipStream_t s);
+ * m in [n, mt), lower triangle only on diagonal tiles, with P(m) the first
+ * kdepth columns of the group buffer tile m: the syrk and gemm tasks of
+ * kdepth / nb consecutive steps in one product (reference pzpotrf.c:58-78;
+ * upper: :100-125).  The same launch copies the panels of the group's first
+ * copy_cols tile columns into A(m, pbase + j), m > pbase + j; an empty
+ * column range with copy_cols > 0 is a copy only.  max_wgs > 0 caps the
+ * grid (its workgroups loop over the jobs), leaving workgroup slots free
+ * for the lookahead stream. */
+int plasma_kern_trailing_update(plasma_enum_t uplo, plasma_desc_t A,
+                                int pbase, int kdepth, int n_first,
+                                int n_end, const void *panel, int pcols,
+                                int copy_cols, int max_wgs,
+                                const int *status, hipStream_t s);
 
 /*
  * Triangular sweeps of potrs (reference pztrsm.c:113-177 and :43-107).

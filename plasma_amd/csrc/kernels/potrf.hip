@@ -465,21 +465,24 @@ __global__ __launch_bounds__(NTHREADS) void potrf_diag_kernel(
 // Job decoders for the block GEMM (tilegemm.hip.h).
 // ---------------------------------------------------------------------------
 
-// Trailing update of step k, tile columns [n_first, ...): tiles (m, n)
-// enumerated column by column, bpt x bpt blocks of 128 per tile.  The solved
-// panel is read from the panel buffer (tile m at panel + (m-k-1) * mb * nb,
-// column major, ld = mb), so both operands have unit row stride for either
-// storage triangle.  The first `ncopy` workgroups copy the panel buffer back
-// into A(m, k), m > k, instead (transposed for upper storage).
+// Trailing update by the panels of a column group: tiles (m, n) of columns
+// [n_first, ...) enumerated column by column, bpt x bpt blocks of 128 per
+// tile, C(m, n) -= P(m) P(n)^T with P(m) the first `kdepth` columns of the
+// group's panel buffer tile m (at panel + (m - pbase - 1) * mb * pcols,
+// column major, ld = mb; its columns [j * nb, (j + 1) * nb) hold the solved
+// panel of tile column pbase + j).  Both operands have unit row stride for
+// either storage triangle.  The first `ncopy` workgroups copy panel columns
+// back into A(m, pbase + j), m > pbase + j, j < ncols, instead (transposed
+// for upper storage).
 template <typename T>
 struct UpdateDecode {
     plasma_desc_t A;
-    int k, n_first, bpt, upper, ncopy;
+    int pbase, pcols, kdepth, ncols, n_first, bpt, upper, ncopy;
     const T *panel;
     __device__ View<T> panel_tile(int m) const
     {
         View<T> v;
-        v.p = const_cast<T *>(panel) + (size_t)(m - k - 1) * A.mb * A.nb;
+        v.p = const_cast<T *>(panel) + (size_t)(m - pbase - 1) * A.mb * pcols;
         v.rs = 1;
         v.cs = A.mb;
         return v;
@@ -487,10 +490,17 @@ struct UpdateDecode {
     __device__ bool operator()(int b, tg::Job<T> &j) const
     {
         if (b < ncopy) {
-            const int m = k + 1 + b;
-            const int mv = plasma_tile_mview(A, m), nb = plasma_tile_mview(A, k);
-            const View<T> D = lower_view<T>(A, upper != 0, m, k);
-            const View<T> S = panel_tile(m);
+            // copy job b: tile column pbase + jc, tile m > pbase + jc
+            int jc = 0, bb = b;
+            while (jc + 1 < ncols && bb >= A.mt - (pbase + jc) - 1) {
+                bb -= A.mt - (pbase + jc) - 1;
+                jc++;
+            }
+            const int col = pbase + jc, m = col + 1 + bb;
+            const int mv = plasma_tile_mview(A, m), nb = plasma_tile_mview(A, col);
+            const View<T> D = lower_view<T>(A, upper != 0, m, col);
+            View<T> S = panel_tile(m);
+            S.p += (size_t)jc * A.nb * A.mb;
             if (!upper) {
                 const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
                 for (int c = wave; c < nb; c += tg::NT / 64)
@@ -520,19 +530,20 @@ struct UpdateDecode {
         j.C = sub(lower_view<T>(A, upper, m, n), r0, c0);
         j.xrows = mv - r0;
         j.yrows = nv - c0;
-        j.K = plasma_tile_mview(A, k);
+        j.K = kdepth;
         j.lower = (m == n);
         j.diag_off = c0 - r0;
         return true;
     }
 };
 
-// Panel solve of step k: P(m) = A(m,k) W_kk^T for m > k into the panel
-// buffer (tile m at panel + (m-k-1) * mb * nb, column major, ld = mb).
+// Panel solve of step k: P(m) = A(m,k) W_kk^T for m > k into column block
+// `pcol` of the panel buffer tile m (at panel + (m - pbase - 1) * mb * pcols
+// + pcol * mb, column major, ld = mb).
 template <typename T>
 struct PanelDecode {
     plasma_desc_t A;
-    int k, bpt, upper;
+    int k, bpt, upper, pbase, pcols, pcol;
     const T *inv;
     T *panel;
     __device__ bool operator()(int b, tg::Job<T> &j) const
@@ -549,7 +560,8 @@ struct PanelDecode {
         W.rs = 1;
         W.cs = A.mb;
         View<T> S;
-        S.p = panel + (size_t)(m - k - 1) * A.mb * A.nb;
+        S.p = panel + (size_t)(m - pbase - 1) * A.mb * pcols +
+              (size_t)pcol * A.mb;
         S.rs = 1;
         S.cs = A.mb;
         j.X = sub(lower_view<T>(A, upper, m, k), r0, 0);
@@ -593,31 +605,32 @@ int launch_potrf_diag(plasma_enum_t uplo, plasma_desc_t A, int k0, int count,
 
 template <typename T>
 int launch_trsm_panel(plasma_enum_t uplo, plasma_desc_t A, int k,
-                      const void *inv, void *panel, const int *status,
-                      hipStream_t s)
+                      const void *inv, void *panel, int pbase, int pcols,
+                      int pcol, const int *status, hipStream_t s)
 {
     const int ntiles = A.mt - k - 1;
     if (ntiles <= 0) return 0;
     const int bpt = (A.mb + tg::BM - 1) / tg::BM;
-    PanelDecode<T> d{A, k, bpt, uplo == PlasmaUpper, (const T *)inv,
-                     (T *)panel};
+    PanelDecode<T> d{A, k, bpt, uplo == PlasmaUpper, pbase, pcols, pcol,
+                     (const T *)inv, (T *)panel};
     if (uplo == PlasmaUpper)
         hipLaunchKernelGGL((tg::kernel<T, true, false, false, false, true,
                                        PanelDecode<T>>),
                            dim3(ntiles * bpt * bpt), dim3(tg::NT), 0, s, d,
-                           status);
+                           ntiles * bpt * bpt, status);
     else
         hipLaunchKernelGGL((tg::kernel<T, true, false, false, true, true,
                                        PanelDecode<T>>),
                            dim3(ntiles * bpt * bpt), dim3(tg::NT), 0, s, d,
-                           status);
+                           ntiles * bpt * bpt, status);
     return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
 }
 
 template <typename T>
-int launch_trailing_update(plasma_enum_t uplo, plasma_desc_t A, int k,
-                           int n_first, int n_end, const void *panel,
-                           int copy_back, const int *status, hipStream_t s)
+int launch_trailing_update(plasma_enum_t uplo, plasma_desc_t A, int pbase,
+                           int kdepth, int n_first, int n_end,
+                           const void *panel, int pcols, int copy_cols,
+                           int max_wgs, const int *status, hipStream_t s)
 {
     n_end = min(n_end, A.mt);
     long nblocks = 0;
@@ -628,22 +641,25 @@ int launch_trailing_update(plasma_enum_t uplo, plasma_desc_t A, int k,
         const long ntiles = ncols * T0 - ncols * (ncols - 1) / 2;
         nblocks = ntiles * bpt * bpt;
     }
-    const int ncopy = copy_back ? max(A.mt - k - 1, 0) : 0;
+    int ncopy = 0;
+    for (int j = 0; j < copy_cols; j++)
+        ncopy += max(A.mt - (pbase + j) - 1, 0);
     nblocks += ncopy;
     if (nblocks == 0) return 0;
     if (nblocks > 0x7fffffffL) return PlasmaErrorNotSupported;
-    UpdateDecode<T> d{A, k, n_first, bpt, uplo == PlasmaUpper, ncopy,
-                      (const T *)panel};
+    const long grid = max_wgs > 0 ? min(nblocks, (long)max_wgs) : nblocks;
+    UpdateDecode<T> d{A, pbase, pcols, kdepth, max(copy_cols, 1), n_first,
+                      bpt, uplo == PlasmaUpper, ncopy, (const T *)panel};
     if (uplo == PlasmaUpper)
         hipLaunchKernelGGL((tg::kernel<T, false, true, true, true, true,
                                        UpdateDecode<T>>),
-                           dim3((unsigned)nblocks), dim3(tg::NT), 0, s, d,
-                           status);
+                           dim3((unsigned)grid), dim3(tg::NT), 0, s, d,
+                           (int)nblocks, status);
     else
         hipLaunchKernelGGL((tg::kernel<T, true, true, true, true, true,
                                        UpdateDecode<T>>),
-                           dim3((unsigned)nblocks), dim3(tg::NT), 0, s, d,
-                           status);
+                           dim3((unsigned)grid), dim3(tg::NT), 0, s, d,
+                           (int)nblocks, status);
     return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
 }
 
@@ -675,26 +691,33 @@ extern "C" int plasma_kern_trtri_diag(plasma_enum_t uplo, plasma_desc_t A,
 
 extern "C" int plasma_kern_trsm_panel(plasma_enum_t uplo, plasma_desc_t A,
                                       int k, const void *inv, void *panel,
+                                      int pbase, int pcols, int pcol,
                                       const int *status, hipStream_t s)
 {
     if (A.precision == PlasmaRealDouble)
-        return launch_trsm_panel<double>(uplo, A, k, inv, panel, status, s);
+        return launch_trsm_panel<double>(uplo, A, k, inv, panel, pbase,
+                                         pcols, pcol, status, s);
     if (A.precision == PlasmaRealFloat)
-        return launch_trsm_panel<float>(uplo, A, k, inv, panel, status, s);
+        return launch_trsm_panel<float>(uplo, A, k, inv, panel, pbase, pcols,
+                                        pcol, status, s);
     return PlasmaErrorNotSupported;
 }
 
 extern "C" int plasma_kern_trailing_update(plasma_enum_t uplo,
-                                           plasma_desc_t A, int k,
-                                           int n_first, int n_end,
-                                           const void *panel, int copy_back,
-                                           const int *status, hipStream_t s)
+                                           plasma_desc_t A, int pbase,
+                                           int kdepth, int n_first,
+                                           int n_end, const void *panel,
+                                           int pcols, int copy_cols,
+                                           int max_wgs, const int *status,
+                                           hipStream_t s)
 {
     if (A.precision == PlasmaRealDouble)
-        return launch_trailing_update<double>(uplo, A, k, n_first, n_end,
-                                              panel, copy_back, status, s);
+        return launch_trailing_update<double>(uplo, A, pbase, kdepth,
+                                              n_first, n_end, panel, pcols,
+                                              copy_cols, max_wgs, status, s);
     if (A.precision == PlasmaRealFloat)
-        return launch_trailing_update<float>(uplo, A, k, n_first, n_end,
-                                             panel, copy_back, status, s);
+        return launch_trailing_update<float>(uplo, A, pbase, kdepth, n_first,
+                                             n_end, panel, pcols, copy_cols,
+                                             max_wgs, status, s);
     return PlasmaErrorNotSupported;
 }

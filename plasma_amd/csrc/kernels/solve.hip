@@ -255,12 +255,12 @@ int launch_scale_panels(plasma_enum_t uplo, plasma_desc_t A, const void *inv,
         hipLaunchKernelGGL((tg::kernel<T, true, false, false, false, false,
                                        ScaleDecode<T>>),
                            dim3((unsigned)nblocks), dim3(tg::NT), 0, s, d,
-                           status);
+                           (int)nblocks, status);
     else
         hipLaunchKernelGGL((tg::kernel<T, true, false, false, true, false,
                                        ScaleDecode<T>>),
                            dim3((unsigned)nblocks), dim3(tg::NT), 0, s, d,
-                           status);
+                           (int)nblocks, status);
     return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
 }
 

@@ -223,17 +223,25 @@ __device__ __forceinline__ void run(const Job<T> &job, T (*sx)[Tile<T>::KC * BM]
 }
 
 // XU / YU: the decoder's X / Y views have unit row stride (else unit k
-// stride); fixed at compile time so each loader has one code path.
+// stride); fixed at compile time so each loader has one code path.  Jobs
+// b = blockIdx.x, blockIdx.x + gridDim.x, ... < njobs: a grid smaller than
+// the job count keeps its workgroups resident and leaves the rest of the
+// chip's workgroup slots to other streams (the jobs are independent, so
+// any grid size is correct).
 template <typename T, bool CROW, bool ACC, bool NEG, bool XU, bool YU,
           typename Decode>
-__global__ __launch_bounds__(NT, 2) void kernel(Decode dec, const int *status)
+__global__ __launch_bounds__(NT, 2) void kernel(Decode dec, int njobs,
+                                                const int *status)
 {
     __shared__ __attribute__((aligned(16))) T sx[2][Tile<T>::KC * BM];
     __shared__ __attribute__((aligned(16))) T sy[2][Tile<T>::KC * BM];
     if (block_should_stop(status)) return;
-    Job<T> job;
-    if (!dec(blockIdx.x, job)) return;
-    run<T, CROW, ACC, NEG, XU, YU>(job, sx, sy);
+    for (int b = blockIdx.x; b < njobs; b += gridDim.x) {
+        Job<T> job;
+        if (!dec(b, job)) continue;
+        // run() ends with a barrier after its last LDS read
+        run<T, CROW, ACC, NEG, XU, YU>(job, sx, sy);
+    }
 }
 
 }  // namespace tg

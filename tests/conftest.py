@@ -33,8 +33,9 @@ def plasma():
 def nb_guard(plasma):
     """Restore the tile size / tuning flags a test changes."""
     import plasma_amd as pa
-    saved = [pa.get_param(p)[1] for p in (pa.PlasmaNb, pa.PlasmaTuning,
-                                         pa.PlasmaLookahead)]
+    params = (pa.PlasmaNb, pa.PlasmaTuning, pa.PlasmaLookahead,
+              pa.PlasmaUpdateDepth)
+    saved = [pa.get_param(p)[1] for p in params]
     yield
-    for p, v in zip((pa.PlasmaNb, pa.PlasmaTuning, pa.PlasmaLookahead), saved):
+    for p, v in zip(params, saved):
         pa.set_param(p, v)

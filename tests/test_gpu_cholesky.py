@@ -172,3 +172,50 @@ def test_dposv_8192(plasma, nb_guard, lookahead):
     F = da.cpu().numpy().T
     ref, _ = _oracle_factor(L, A, nb)
     assert U.factor_error(F, ref, True) <= 1e-12
+
+
+@pytest.mark.parametrize("uplo", [L, Up])
+@pytest.mark.parametrize("depth", [1, 2, 3, 4])
+@pytest.mark.parametrize("lookahead", [0, 1])
+@pytest.mark.parametrize("n,nb", [(1000, 128), (1152, 128), (300, 64)])
+def test_dpotrf_update_depth(plasma, nb_guard, uplo, depth, lookahead, n,
+                             nb):
+    """Trailing updates by groups of 1-4 panels (K = depth * nb), with and
+    without lookahead, including a short last tile row (n % nb != 0) and
+    tile counts that are not a multiple of the group size: the factor
+    against the oracle and the untouched triangle."""
+    pa.set_param(pa.PlasmaTuning, pa.PlasmaDisabled)
+    pa.set_param(pa.PlasmaNb, nb)
+    pa.set_param(pa.PlasmaLookahead, lookahead)
+    pa.set_param(pa.PlasmaUpdateDepth, depth)
+    A = U.tester_spd(n, n)
+    a = A.copy(order="F")
+    assert pa.potrf(uplo, n, a, n) == 0
+    ref, rinfo = _oracle_factor(uplo, A, nb)
+    assert rinfo == 0
+    assert U.factor_error(a, ref, uplo == L) <= 1e-12
+    other = np.triu(A, 1) if uplo == L else np.tril(A, -1)
+    got = np.triu(a, 1) if uplo == L else np.tril(a, -1)
+    assert np.array_equal(got, other)
+
+
+@pytest.mark.parametrize("depth", [1, 3])
+def test_dposv_8192_depth(plasma, nb_guard, depth):
+    """n = 8192 with lookahead at update depths other than the default."""
+    import torch
+    n, nb, nrhs = 8192, 256, 64
+    pa.set_param(pa.PlasmaTuning, pa.PlasmaDisabled)
+    pa.set_param(pa.PlasmaNb, nb)
+    pa.set_param(pa.PlasmaLookahead, 1)
+    pa.set_param(pa.PlasmaUpdateDepth, depth)
+    A = U.ggt_spd(n, seed=13)
+    B = U.rhs(n, nrhs, seed=14)
+    da = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()
+    db = torch.from_numpy(np.ascontiguousarray(B.T)).cuda()
+    torch.cuda.synchronize()
+    assert pa.posv(L, n, nrhs, da, n, db, n) == 0
+    X = db.cpu().numpy().T
+    assert U.scaled_residual(A, X, B) < 10 * U.EPS64
+    F = da.cpu().numpy().T
+    ref, _ = _oracle_factor(L, A, nb)
+    assert U.factor_error(F, ref, True) <= 1e-12

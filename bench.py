@@ -131,6 +131,10 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=16384,
                     help="order of the CPU-baseline sample (0 skips it)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="after the solves, copy MIB MiB device to device "
+                         "once (a kernel of known traffic for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -393,6 +397,15 @@ def main():
         "kernels": classes(prof_m, args.steps),
         "plain_ggt": ggt,
     }
+    if args.calibrate_copy > 0:
+        nel = args.calibrate_copy * (1 << 20) // 8
+        src = torch.ones(nel, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        line["calibration_copy_bytes"] = 2 * nel * 8
+        del src, dst
     if args.cpu_n > 0:
         print("bench: cpu baseline n=%d" % args.cpu_n, file=sys.stderr,
               flush=True)

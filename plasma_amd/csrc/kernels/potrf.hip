@@ -616,13 +616,13 @@ int launch_trsm_panel(plasma_enum_t uplo, plasma_desc_t A, int k,
     if (uplo == PlasmaUpper)
         hipLaunchKernelGGL((tg::kernel<T, true, false, false, false, true,
                                        PanelDecode<T>>),
-                           dim3(ntiles * bpt * bpt), dim3(tg::NT), 0, s, d,
-                           ntiles * bpt * bpt, status);
+                           dim3(tg::xcd_grid(ntiles * bpt * bpt)),
+                           dim3(tg::NT), 0, s, d, ntiles * bpt * bpt, status);
     else
         hipLaunchKernelGGL((tg::kernel<T, true, false, false, true, true,
                                        PanelDecode<T>>),
-                           dim3(ntiles * bpt * bpt), dim3(tg::NT), 0, s, d,
-                           ntiles * bpt * bpt, status);
+                           dim3(tg::xcd_grid(ntiles * bpt * bpt)),
+                           dim3(tg::NT), 0, s, d, ntiles * bpt * bpt, status);
     return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
 }
 
@@ -647,7 +647,9 @@ int launch_trailing_update(plasma_enum_t uplo, plasma_desc_t A, int pbase,
     nblocks += ncopy;
     if (nblocks == 0) return 0;
     if (nblocks > 0x7fffffffL) return PlasmaErrorNotSupported;
-    const long grid = max_wgs > 0 ? min(nblocks, (long)max_wgs) : nblocks;
+    const long grid = max_wgs > 0 ? min((long)tg::xcd_grid((int)nblocks),
+                                        (long)max_wgs)
+                                  : (long)tg::xcd_grid((int)nblocks);
     UpdateDecode<T> d{A, pbase, pcols, kdepth, max(copy_cols, 1), n_first,
                       bpt, uplo == PlasmaUpper, ncopy, (const T *)panel};
     if (uplo == PlasmaUpper)

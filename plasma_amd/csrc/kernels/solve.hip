@@ -254,13 +254,13 @@ int launch_scale_panels(plasma_enum_t uplo, plasma_desc_t A, const void *inv,
     if (uplo == PlasmaUpper)
         hipLaunchKernelGGL((tg::kernel<T, true, false, false, false, false,
                                        ScaleDecode<T>>),
-                           dim3((unsigned)nblocks), dim3(tg::NT), 0, s, d,
-                           (int)nblocks, status);
+                           dim3(tg::xcd_grid((int)nblocks)), dim3(tg::NT), 0,
+                           s, d, (int)nblocks, status);
     else
         hipLaunchKernelGGL((tg::kernel<T, true, false, false, true, false,
                                        ScaleDecode<T>>),
-                           dim3((unsigned)nblocks), dim3(tg::NT), 0, s, d,
-                           (int)nblocks, status);
+                           dim3(tg::xcd_grid((int)nblocks)), dim3(tg::NT), 0,
+                           s, d, (int)nblocks, status);
     return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
 }
 

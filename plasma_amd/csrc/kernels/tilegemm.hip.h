@@ -222,12 +222,26 @@ __device__ __forceinline__ void run(const Job<T> &job, T (*sx)[Tile<T>::KC * BM]
             }
 }
 
+// Workgroups are dispatched to the eight XCDs of the chip round robin
+// (workgroup b to XCD b % 8), and each XCD has its own L2.  Decoders number
+// their jobs so that neighbours share operands (the blocks of one tile,
+// the tiles of one column), so workgroup b runs job
+//     xcd_job(b) = (b % 8) * chunk + b / 8,   chunk = ceil(njobs / 8),
+// which gives every XCD one contiguous range of jobs.  Grids are padded to
+// a multiple of 8 (8 * chunk); indices past njobs do nothing.
+constexpr int NXCD = 8;
+
+__host__ __device__ __forceinline__ int xcd_grid(int njobs)
+{
+    return (njobs + NXCD - 1) / NXCD * NXCD;
+}
+
 // XU / YU: the decoder's X / Y views have unit row stride (else unit k
-// stride); fixed at compile time so each loader has one code path.  Jobs
-// b = blockIdx.x, blockIdx.x + gridDim.x, ... < njobs: a grid smaller than
-// the job count keeps its workgroups resident and leaves the rest of the
-// chip's workgroup slots to other streams (the jobs are independent, so
-// any grid size is correct).
+// stride); fixed at compile time so each loader has one code path.  The
+// workgroups loop over b = blockIdx.x, blockIdx.x + gridDim.x, ... below
+// the padded job count: a grid smaller than that keeps its workgroups
+// resident and leaves the rest of the chip's workgroup slots to other
+// streams (the jobs are independent, so any grid size is correct).
 template <typename T, bool CROW, bool ACC, bool NEG, bool XU, bool YU,
           typename Decode>
 __global__ __launch_bounds__(NT, 2) void kernel(Decode dec, int njobs,
@@ -236,9 +250,11 @@ __global__ __launch_bounds__(NT, 2) void kernel(Decode dec, int njobs,
     __shared__ __attribute__((aligned(16))) T sx[2][Tile<T>::KC * BM];
     __shared__ __attribute__((aligned(16))) T sy[2][Tile<T>::KC * BM];
     if (block_should_stop(status)) return;
-    for (int b = blockIdx.x; b < njobs; b += gridDim.x) {
+    const int padded = xcd_grid(njobs), chunk = padded / NXCD;
+    for (int b = blockIdx.x; b < padded; b += gridDim.x) {
+        const int jb = (b % NXCD) * chunk + b / NXCD;
         Job<T> job;
-        if (!dec(b, job)) continue;
+        if (jb >= njobs || !dec(jb, job)) continue;
         // run() ends with a barrier after its last LDS read
         run<T, CROW, ACC, NEG, XU, YU>(job, sx, sy);
     }

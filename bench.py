@@ -207,12 +207,13 @@ class Bench:
         lib.plasma_profile_enable(0)
         return out
 
-    def timed(self, step, label):
+    def timed(self, step, label, profile=True):
         torch, args = self.torch, self.args
         for _ in range(args.warmup):
             step()
         print("bench: %s warmup done" % label, file=sys.stderr, flush=True)
-        self.prof_reset()
+        if profile:
+            self.prof_reset()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for i in range(args.steps):
@@ -221,9 +222,9 @@ class Bench:
                   flush=True)
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / args.steps
-        return ms, self.prof_read(args.steps)
+        return ms, (self.prof_read(args.steps) if profile else {})
 
-    def plain(self, A0, B0, label):
+    def plain(self, A0, B0, label, profile=True):
         """Plain FP64 dposv on device-resident LAPACK arrays."""
         torch, pa, lib = self.torch, self.pa, self.lib
         n, nrhs, nb = A0.shape[0], B0.shape[0], self.args.nb
@@ -253,12 +254,12 @@ class Bench:
             if status != 0:
                 raise SystemExit("plasma dposv failed: %d" % status)
 
-        ms, prof = self.timed(step, label)
+        ms, prof = self.timed(step, label, profile)
         del Aout
         res = scaled_residual(A0, X, B0)
         return ms, prof, res
 
-    def mixed(self, A0, B0, label):
+    def mixed(self, A0, B0, label, profile=True):
         """plasma_dsposv on device-resident LAPACK arrays."""
         torch, pa, lib = self.torch, self.pa, self.lib
         n, nrhs = A0.shape[0], B0.shape[0]
@@ -274,7 +275,7 @@ class Bench:
                 raise SystemExit("plasma dsposv failed: %d" % rc)
             iters.append(it.value)
 
-        ms, prof = self.timed(step, label)
+        ms, prof = self.timed(step, label, profile)
         res = scaled_residual(A0, X, B0)
         return ms, prof, res, iters[-1]
 
@@ -292,7 +293,22 @@ def classes(prof, steps):
     return out
 
 
-def roofline(prof, kernel, peak):
+def pmc_traffic(name):
+    """HBM bytes per launch of a kernel, from the summary of the two
+    rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE, each scaled by the same
+    counter's reading for a copy of known size) kept in profiles/<name>.md
+    as a line 'traffic_bytes_per_launch: <bytes>'; None when absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", name + ".md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", text, re.M)
+    return float(m.group(1)) if m else None
+
+
+def roofline(prof, kernel, peak, traffic=None):
     p = prof.get("trailing_update")
     if not p:
         return None
@@ -302,7 +318,7 @@ def roofline(prof, kernel, peak):
             "peak": peak / 1e12, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4),
             "avg_launch_ms": round(p["ms"] / p["launches"], 4),
-            "traffic": None}
+            "traffic": traffic}
 
 
 def main():
@@ -323,12 +339,30 @@ def main():
 
     sweep = {}
     head = None
+    graphs = None
     for kappa in kappas:
         A0, _ = householder_spd(n, kappa, seed=0)
         torch.cuda.synchronize()
         tag = "kappa=%.0e" % kappa
         ms_m, prof_m, res_m, iters = bench.mixed(A0, B0, "mixed " + tag)
         ms_p, prof_p, res_p = bench.plain(A0, B0, "plain " + tag)
+        if kappa == kappas[-1]:
+            # the same two solves replaying captured factorization graphs
+            # (the profiler is off: it times launches one by one)
+            bench.pa.set_param(bench.pa.PlasmaGraphCapture,
+                               bench.pa.PlasmaEnabled)
+            gm, _, gres_m, _ = bench.mixed(A0, B0, "mixed graphs " + tag,
+                                           profile=False)
+            gp, _, gres_p = bench.plain(A0, B0, "plain graphs " + tag,
+                                        profile=False)
+            bench.pa.set_param(bench.pa.PlasmaGraphCapture,
+                               bench.pa.PlasmaDisabled)
+            graphs = {"mixed_ms": round(gm, 3),
+                      "mixed_tflops": round(fl / (gm * 1e-3) / 1e12, 3),
+                      "mixed_residual_eps": round(gres_m, 4),
+                      "plain_ms": round(gp, 3),
+                      "plain_tflops": round(fl / (gp * 1e-3) / 1e12, 3),
+                      "plain_residual_eps": round(gres_p, 4)}
         del A0
         sweep[tag] = {
             "mixed_tflops": round(fl / (ms_m * 1e-3) / 1e12, 3),
@@ -393,9 +427,12 @@ def main():
         "kappa_sweep": sweep,
         "roofline": roofline(prof_m, "trailing_update_kernel<float> (FP32 "
                              "MFMA syrk+gemm of the FP32 factor)",
-                             FP32_MFMA_PEAK),
+                             FP32_MFMA_PEAK,
+                             pmc_traffic("trailing_update_fp32_dsposv_"
+                                         "n32768_pmc_traffic")),
         "kernels": classes(prof_m, args.steps),
         "plain_ggt": ggt,
+        "graph_mode": graphs,
     }
     if args.calibrate_copy > 0:
         nel = args.calibrate_copy * (1 << 20) // 8

@@ -27,6 +27,8 @@
  * so the critical path potrf -> trsm -> next column overlaps the bulk
  * update.  The streams are joined again before returning.
  */
+#include <string.h>
+
 #include "internal.h"
 
 static int issue(int err, plasma_sequence_t *sequence,
@@ -138,42 +140,20 @@ static int factor_group(const group_t *g, int c0, int w, void *panel,
     return 0;
 }
 
-void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A, void *inv,
-                    plasma_sequence_t *sequence, plasma_request_t *request)
+/* The launch sequence of the factorization: on return every launch is
+ * enqueued (main stream joined with the panel stream).  Returns non-zero
+ * when a launch failed (the sequence is marked).  ev: three events the
+ * caller owns for the duration of the call (and of a capture). */
+static int pxpotrf_launch(plasma_enum_t uplo, plasma_desc_t A, void *inv,
+                          void *panel[2], int lookahead, int *st,
+                          hipEvent_t ev[3], plasma_sequence_t *sequence,
+                          plasma_request_t *request)
 {
-    if (sequence->status != PlasmaSuccess)
-        return;
     plasma_context_t *ctx = plasma_context_self();
-    int *st = plasma_sequence_dstatus(sequence, request);
     hipStream_t s0 = ctx->stream, s1 = ctx->panel_stream;
     const int mt = A.mt;
-    size_t es = plasma_element_size(A.precision);
     const int gw = imax(1, imin(ctx->group, mt));
-    const int lookahead = ctx->lookahead > 0 && mt > gw;
-
-    void *own_inv = NULL;
-    if (inv == NULL) {
-        inv = own_inv = plasma_dev_alloc(plasma_inv_bytes(A));
-        if (inv == NULL) {
-            plasma_request_fail(sequence, request, PlasmaErrorOutOfMemory);
-            return;
-        }
-    }
     group_t g = {uplo, A, inv, st, gw * A.nb};
-    /* panel buffers: the solved panels of group q live in panel[q % 2]
-     * until the rest of group q's update has copied them into A */
-    const size_t pbytes = (size_t)imax(mt - 1, 1) * A.mb * g.pcols * es;
-    void *panel[2] = {NULL, NULL};
-    panel[0] = plasma_dev_alloc(pbytes);
-    panel[1] = lookahead ? plasma_dev_alloc(pbytes) : panel[0];
-    if (panel[0] == NULL || panel[1] == NULL) {
-        if (panel[1] != panel[0])
-            plasma_dev_free(panel[1]);
-        plasma_dev_free(panel[0]);
-        plasma_dev_free(own_inv);
-        plasma_request_fail(sequence, request, PlasmaErrorOutOfMemory);
-        return;
-    }
 
     if (!lookahead) {
         for (int c0 = 0; c0 < mt; c0 += gw) {
@@ -181,30 +161,12 @@ void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A, void *inv,
             if (factor_group(&g, c0, w, panel[0], s0, sequence, request) ||
                 issue(k_update(&g, c0, w, c0 + w, mt, panel[0], w, 0, s0),
                       sequence, request))
-                break;
+                return 1;
         }
-        plasma_dev_free(panel[0]);
-        plasma_dev_free(own_inv);
-        return;
+        return 0;
     }
 
-    hipEvent_t ev_fork, ev_col, ev_panel;
-    if (plasma_hip_check(hipEventCreateWithFlags(&ev_fork,
-                                                 hipEventDisableTiming),
-                         "hipEventCreate") ||
-        plasma_hip_check(hipEventCreateWithFlags(&ev_col,
-                                                 hipEventDisableTiming),
-                         "hipEventCreate") ||
-        plasma_hip_check(hipEventCreateWithFlags(&ev_panel,
-                                                 hipEventDisableTiming),
-                         "hipEventCreate")) {
-        plasma_request_fail(sequence, request, PlasmaErrorInternal);
-        plasma_dev_free(panel[1]);
-        plasma_dev_free(panel[0]);
-        plasma_dev_free(own_inv);
-        return;
-    }
-
+    hipEvent_t ev_fork = ev[0], ev_col = ev[1], ev_panel = ev[2];
     /* No cap on the bulk update's grid (0): leaving workgroup slots to the
      * lookahead stream starved the panel solve more than it helped the
      * diagonal factorization. */
@@ -247,12 +209,203 @@ void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A, void *inv,
     /* join */
     hipEventRecord(ev_panel, s1);
     hipStreamWaitEvent(s0, ev_panel, 0);
-    hipEventDestroy(ev_fork);
-    hipEventDestroy(ev_col);
-    hipEventDestroy(ev_panel);
-    plasma_dev_free(panel[1]);
+    return err;
+}
+
+/* Workspace of one factorization: the inverses (when the caller passes
+ * none) and the panel buffers (two with lookahead).  Returns non-zero when
+ * out of memory, with nothing left allocated. */
+static int pxpotrf_buffers(plasma_desc_t A, void *inv, int lookahead,
+                           void **own_inv, void *panel[2])
+{
+    plasma_context_t *ctx = plasma_context_self();
+    const int gw = imax(1, imin(ctx->group, A.mt));
+    const size_t pbytes = (size_t)imax(A.mt - 1, 1) * A.mb * gw * A.nb *
+                          plasma_element_size(A.precision);
+    *own_inv = NULL;
+    panel[0] = panel[1] = NULL;
+    if (inv == NULL && (*own_inv = plasma_dev_alloc(plasma_inv_bytes(A))) ==
+                           NULL)
+        return 1;
+    panel[0] = plasma_dev_alloc(pbytes);
+    panel[1] = lookahead ? plasma_dev_alloc(pbytes) : panel[0];
+    if (panel[0] == NULL || panel[1] == NULL) {
+        if (panel[1] != panel[0])
+            plasma_dev_free(panel[1]);
+        plasma_dev_free(panel[0]);
+        plasma_dev_free(*own_inv);
+        *own_inv = NULL;
+        return 1;
+    }
+    return 0;
+}
+
+static void pxpotrf_free(void *own_inv, void *panel[2])
+{
+    if (panel[1] != panel[0])
+        plasma_dev_free(panel[1]);
     plasma_dev_free(panel[0]);
     plasma_dev_free(own_inv);
+}
+
+static int events_create(hipEvent_t ev[3])
+{
+    for (int i = 0; i < 3; i++)
+        if (plasma_hip_check(hipEventCreateWithFlags(&ev[i],
+                                                     hipEventDisableTiming),
+                             "hipEventCreate")) {
+            while (i-- > 0)
+                hipEventDestroy(ev[i]);
+            return 1;
+        }
+    return 0;
+}
+
+static void events_destroy(hipEvent_t ev[3])
+{
+    for (int i = 0; i < 3; i++)
+        hipEventDestroy(ev[i]);
+}
+
+static void graph_drop(plasma_graph_entry_t *e)
+{
+    if (!e->used)
+        return;
+    hipGraphExecDestroy(e->exec);
+    pxpotrf_free(e->own_inv, e->panel);
+    memset(e, 0, sizeof(*e));
+}
+
+void plasma_graph_release_all(plasma_context_t *ctx)
+{
+    hipStreamSynchronize(ctx->panel_stream);
+    hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i < PLASMA_GRAPHS; i++)
+        graph_drop(&ctx->graphs[i]);
+}
+
+/*
+ * Graph mode (PlasmaGraphCapture enabled): the launch sequence of a
+ * factorization is captured once into a hipGraph (the two streams fork
+ * and join inside the capture) and replayed on later calls with the same
+ * descriptor, inverse workspace and schedule; the graph owns its panel
+ * buffers.  Its launches read and write the context's graph status word,
+ * which is loaded from the sequence's word before a replay and stored back
+ * after it, so one graph serves every sequence.
+ */
+static plasma_graph_entry_t *graph_get(plasma_enum_t uplo, plasma_desc_t A,
+                                       void *inv, int lookahead,
+                                       plasma_sequence_t *sequence,
+                                       plasma_request_t *request)
+{
+    plasma_context_t *ctx = plasma_context_self();
+    plasma_graph_entry_t *e = NULL, *victim = &ctx->graphs[0];
+    for (int i = 0; i < PLASMA_GRAPHS; i++) {
+        plasma_graph_entry_t *c = &ctx->graphs[i];
+        if (c->used && c->uplo == (int)uplo && c->inv_arg == inv &&
+            c->lookahead == lookahead && c->group == ctx->group &&
+            memcmp(&c->A, &A, sizeof(A)) == 0)
+            e = c;
+        if (!c->used || (victim->used && c->stamp < victim->stamp))
+            victim = c;
+    }
+    if (e != NULL)
+        return e;
+
+    /* capture a new graph into the least recently used entry */
+    if (victim->used) {
+        hipStreamSynchronize(ctx->panel_stream);
+        hipStreamSynchronize(ctx->stream);
+        graph_drop(victim);
+    }
+    e = victim;
+    void *own_inv, *panel[2];
+    if (pxpotrf_buffers(A, inv, lookahead, &own_inv, panel)) {
+        plasma_request_fail(sequence, request, PlasmaErrorOutOfMemory);
+        return NULL;
+    }
+    hipEvent_t ev[3];
+    if (events_create(ev)) {
+        pxpotrf_free(own_inv, panel);
+        plasma_request_fail(sequence, request, PlasmaErrorInternal);
+        return NULL;
+    }
+    hipStream_t s0 = ctx->stream;
+    int *gword = ctx->d_status + PLASMA_STATUS_SLOTS;
+    hipGraph_t graph = NULL;
+    int err = plasma_hip_check(
+        hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed),
+        "hipStreamBeginCapture");
+    if (!err) {
+        int lerr = pxpotrf_launch(uplo, A, inv != NULL ? inv : own_inv, panel,
+                                  lookahead, gword, ev, sequence, request);
+        err = plasma_hip_check(hipStreamEndCapture(s0, &graph),
+                               "hipStreamEndCapture") || lerr;
+    }
+    if (!err)
+        err = plasma_hip_check(hipGraphInstantiate(&e->exec, graph, NULL,
+                                                   NULL, 0),
+                               "hipGraphInstantiate");
+    if (graph != NULL)
+        hipGraphDestroy(graph);
+    events_destroy(ev);
+    if (err) {
+        pxpotrf_free(own_inv, panel);
+        plasma_request_fail(sequence, request, PlasmaErrorInternal);
+        return NULL;
+    }
+    e->used = 1;
+    e->A = A;
+    e->uplo = uplo;
+    e->inv_arg = inv;
+    e->lookahead = lookahead;
+    e->group = ctx->group;
+    e->own_inv = own_inv;
+    e->panel[0] = panel[0];
+    e->panel[1] = panel[1];
+    return e;
+}
+
+void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A, void *inv,
+                    plasma_sequence_t *sequence, plasma_request_t *request)
+{
+    if (sequence->status != PlasmaSuccess)
+        return;
+    plasma_context_t *ctx = plasma_context_self();
+    int *st = plasma_sequence_dstatus(sequence, request);
+    hipStream_t s0 = ctx->stream;
+    const int gw = imax(1, imin(ctx->group, A.mt));
+    const int lookahead = ctx->lookahead > 0 && A.mt > gw;
+
+    if (ctx->graph_capture == PlasmaEnabled && !ctx->profile) {
+        plasma_graph_entry_t *e = graph_get(uplo, A, inv, lookahead,
+                                            sequence, request);
+        if (e == NULL)
+            return;
+        e->stamp = ++ctx->graph_clock;
+        int *gword = ctx->d_status + PLASMA_STATUS_SLOTS;
+        if (issue(plasma_kern_copy_int(gword, st, s0), sequence, request) ||
+            plasma_hip_check(hipGraphLaunch(e->exec, s0), "hipGraphLaunch") ||
+            issue(plasma_kern_copy_int(st, gword, s0), sequence, request))
+            plasma_request_fail(sequence, request, PlasmaErrorInternal);
+        return;
+    }
+
+    void *own_inv, *panel[2];
+    if (pxpotrf_buffers(A, inv, lookahead, &own_inv, panel)) {
+        plasma_request_fail(sequence, request, PlasmaErrorOutOfMemory);
+        return;
+    }
+    hipEvent_t ev[3];
+    if (events_create(ev)) {
+        pxpotrf_free(own_inv, panel);
+        plasma_request_fail(sequence, request, PlasmaErrorInternal);
+        return;
+    }
+    pxpotrf_launch(uplo, A, inv != NULL ? inv : own_inv, panel, lookahead, st,
+                   ev, sequence, request);
+    events_destroy(ev);
+    pxpotrf_free(own_inv, panel);
 }
 
 /*

@@ -104,14 +104,15 @@ int plasma_init(void)
         return PlasmaErrorEnvironment;
     ctx->owns_streams = 1;
 
+    /* one word per sequence slot, plus the word of the cached graphs */
     if (plasma_hip_check(hipMalloc((void **)&ctx->d_status,
-                                   PLASMA_STATUS_SLOTS * sizeof(int)),
+                                   (PLASMA_STATUS_SLOTS + 1) * sizeof(int)),
                          "hipMalloc(status)") ||
         plasma_hip_check(hipMalloc((void **)&ctx->d_scalars,
                                    PLASMA_SCALARS * sizeof(double)),
                          "hipMalloc(scalars)"))
         return PlasmaErrorOutOfMemory;
-    if (plasma_kern_fill_int(ctx->d_status, 0, PLASMA_STATUS_SLOTS,
+    if (plasma_kern_fill_int(ctx->d_status, 0, PLASMA_STATUS_SLOTS + 1,
                              ctx->stream) != 0 ||
         plasma_hip_check(hipStreamSynchronize(ctx->stream), "init sync"))
         return PlasmaErrorEnvironment;
@@ -127,6 +128,7 @@ int plasma_finalize(void)
     plasma_context_t *ctx = &plasma_context_g;
     hipStreamSynchronize(ctx->stream);
     hipStreamSynchronize(ctx->panel_stream);
+    plasma_graph_release_all(ctx);
     plasma_pool_release_all(ctx);
     hipFree(ctx->d_status);
     hipFree(ctx->d_scalars);

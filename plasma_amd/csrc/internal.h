@@ -35,6 +35,20 @@ typedef struct {
     int    in_use;
 } plasma_pool_entry_t;
 
+/* Cached factorization graphs (compute/pxpotrf.c). */
+#define PLASMA_GRAPHS 4
+typedef struct {
+    int used;
+    unsigned long stamp;
+    plasma_desc_t A;          /* the key: descriptor, uplo, inverse   */
+    int uplo;                 /* workspace argument and schedule      */
+    const void *inv_arg;
+    int lookahead, group;
+    hipGraphExec_t exec;
+    void *own_inv;            /* buffers the graph's launches address */
+    void *panel[2];
+} plasma_graph_entry_t;
+
 /* profiler records (control/profile.c) */
 #define PLASMA_PROF_MAX 16384
 typedef struct {
@@ -77,6 +91,12 @@ struct plasma_context_s {
     /* workspace pool */
     plasma_pool_entry_t pool[PLASMA_POOL_ENTRIES];
 
+    /* factorization graphs; their launches use the status word
+     * d_status[PLASMA_STATUS_SLOTS], copied from and back to the
+     * sequence's word around each replay */
+    plasma_graph_entry_t graphs[PLASMA_GRAPHS];
+    unsigned long graph_clock;
+
     /* kernel profiler */
     int profile;
     int nprof;
@@ -101,6 +121,9 @@ int plasma_hip_check(hipError_t err, const char *what);
 /* async.c: the device status word of a sequence (allocated on first use) */
 int *plasma_sequence_dstatus(plasma_sequence_t *sequence,
                              plasma_request_t *request);
+
+/* compute/pxpotrf.c: drop every cached graph (streams drained first) */
+void plasma_graph_release_all(plasma_context_t *ctx);
 
 /* workspace.c: device memory from the pool */
 void *plasma_dev_alloc(size_t bytes);

@@ -114,6 +114,17 @@ int launch_layout(plasma_enum_t region, void *pA, int lda, plasma_desc_t A,
 
 }  // namespace
 
+__global__ void copy_int_kernel(int *dst, const int *src)
+{
+    if (threadIdx.x == 0) *dst = *src;
+}
+
+extern "C" int plasma_kern_copy_int(int *dst, const int *src, hipStream_t s)
+{
+    hipLaunchKernelGGL(copy_int_kernel, dim3(1), dim3(64), 0, s, dst, src);
+    return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
+}
+
 extern "C" int plasma_kern_fill_int(int *p, int v, int n, hipStream_t s)
 {
     if (n <= 0) return 0;

@@ -25,6 +25,8 @@ extern "C" {
 
 /* Fill n ints with v (a kernel, never hipMemset: safe under graph capture). */
 int plasma_kern_fill_int(int *p, int v, int n, hipStream_t s);
+/* *dst = *src on the stream (one int; a kernel, capture-safe). */
+int plasma_kern_copy_int(int *dst, const int *src, hipStream_t s);
 
 /*
  * LAPACK column-major <-> tile layout (reference compute/pztr2desc.c,

@@ -34,7 +34,7 @@ def nb_guard(plasma):
     """Restore the tile size / tuning flags a test changes."""
     import plasma_amd as pa
     params = (pa.PlasmaNb, pa.PlasmaTuning, pa.PlasmaLookahead,
-              pa.PlasmaUpdateDepth)
+              pa.PlasmaUpdateDepth, pa.PlasmaGraphCapture)
     saved = [pa.get_param(p)[1] for p in params]
     yield
     for p, v in zip(params, saved):

@@ -32,6 +32,18 @@ int plasma_profile_enable(int on);
 int plasma_profile_query(int kernel_class, double *milliseconds,
                          double *flops, double *bytes, long *launches);
 
+/*
+ * Autotuned schedules (no counterpart in the reference's C interface; it
+ * replaces the Lua tables behind plasma_tune_potrf, control/tuning.c:
+ * 367-374).  Replaces the table read at plasma_init with the file at path
+ * (NULL: no table).  Lines "potrf <d|s> <n_max> <nb> <update_depth>
+ * <lookahead>"; with PlasmaTuning enabled a factorization of order n takes
+ * nb, update depth and lookahead from the first row of its precision with
+ * n <= n_max (else the last one).  Returns PlasmaErrorIllegalValue when the
+ * file cannot be read.
+ */
+int plasma_tuning_load(const char *path);
+
 #ifdef __cplusplus
 }
 #endif

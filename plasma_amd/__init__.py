@@ -117,6 +117,7 @@ PROTOTYPES.update({
     "plasma_omp_dlag2s": (None, [_D, _D, _PS, _PR]),
     "plasma_omp_slag2d": (None, [_D, _D, _PS, _PR]),
     "plasma_profile_enable": (c_int, [c_int]),
+    "plasma_tuning_load": (c_int, [ctypes.c_char_p]),
     "plasma_profile_query": (c_int, [c_int, ctypes.POINTER(c_double),
                                      ctypes.POINTER(c_double),
                                      ctypes.POINTER(c_double),

@@ -73,7 +73,7 @@ def build(verbose=False):
     objs = [_obj(s) for s in hip + c]
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp]
-         + objs + ["-L" + os.path.join(ROCM, "lib"), "-lamdhip64",
+         + objs + ["-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-ldl",
                    "-Wl,-rpath," + os.path.join(ROCM, "lib")])
     os.replace(tmp, LIB)
     return LIB

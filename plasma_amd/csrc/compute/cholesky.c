@@ -73,12 +73,11 @@ static void arg_stage_out(lapack_arg_t *a, int copy_back)
 }
 
 /* Tile size for a factorization of order n (reference: plasma_tune_potrf,
- * control/tuning.c:367-374, a no-op without its Lua tables).  With tuning
- * enabled the autotuned table decides; otherwise the context nb is used. */
+ * control/tuning.c:367-374): with tuning enabled the autotuned table, when
+ * present, sets the context's nb (and schedule) first. */
 static int tuned_nb(plasma_context_t *ctx, plasma_enum_t precision, int n)
 {
-    (void)precision;
-    (void)n;
+    plasma_tune_potrf(ctx, precision, n);
     return ctx->nb;
 }
 

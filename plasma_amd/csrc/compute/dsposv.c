@@ -262,6 +262,9 @@ int plasma_dsposv(plasma_enum_t uplo, int n, int nrhs, double *pA, int lda,
     if (imin(n, nrhs) == 0)
         return PlasmaSuccess;
 
+    /* reference zcposv.c:160-161: tuned with the single-precision table */
+    if (ctx->tuning)
+        plasma_tune_potrf(ctx, PlasmaRealFloat, n);
     const int nb = ctx->nb;
     plasma_desc_t A, B, X, R, As, Xs;
     plasma_desc_t *all[6] = {&A, &B, &X, &R, &As, &Xs};

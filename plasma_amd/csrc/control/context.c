@@ -118,6 +118,7 @@ int plasma_init(void)
         return PlasmaErrorEnvironment;
 
     plasma_initialized_g = 1;
+    plasma_tuning_init(ctx);
     return PlasmaSuccess;
 }
 

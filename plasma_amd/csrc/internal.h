@@ -35,6 +35,12 @@ typedef struct {
     int    in_use;
 } plasma_pool_entry_t;
 
+/* Autotuned schedule table (control/tuning.c). */
+#define PLASMA_TUNE_ROWS 64
+typedef struct {
+    int precision, n_max, nb, depth, lookahead;
+} plasma_tune_row_t;
+
 /* Cached factorization graphs (compute/pxpotrf.c). */
 #define PLASMA_GRAPHS 4
 typedef struct {
@@ -97,6 +103,10 @@ struct plasma_context_s {
     plasma_graph_entry_t graphs[PLASMA_GRAPHS];
     unsigned long graph_clock;
 
+    /* autotuned schedules */
+    plasma_tune_row_t tune[PLASMA_TUNE_ROWS];
+    int ntune;
+
     /* kernel profiler */
     int profile;
     int nprof;
@@ -121,6 +131,11 @@ int plasma_hip_check(hipError_t err, const char *what);
 /* async.c: the device status word of a sequence (allocated on first use) */
 int *plasma_sequence_dstatus(plasma_sequence_t *sequence,
                              plasma_request_t *request);
+
+/* control/tuning.c: load the table at init; apply it to a factorization
+ * of order n (sets nb, update depth and lookahead in the context) */
+void plasma_tuning_init(plasma_context_t *ctx);
+void plasma_tune_potrf(plasma_context_t *ctx, plasma_enum_t precision, int n);
 
 /* compute/pxpotrf.c: drop every cached graph (streams drained first) */
 void plasma_graph_release_all(plasma_context_t *ctx);

@@ -36,6 +36,9 @@ def nb_guard(plasma):
     params = (pa.PlasmaNb, pa.PlasmaTuning, pa.PlasmaLookahead,
               pa.PlasmaUpdateDepth, pa.PlasmaGraphCapture)
     saved = [pa.get_param(p)[1] for p in params]
+    # a test that sets nb means that nb: the autotuned table (on by default,
+    # as in the reference) is switched off unless the test enables it
+    pa.set_param(pa.PlasmaTuning, pa.PlasmaDisabled)
     yield
     for p, v in zip(params, saved):
         pa.set_param(p, v)

@@ -17,5 +17,6 @@
 #include "plasma_s.h"
 #include "plasma_ds.h"
 #include "plasma_mi355x.h"
+#include "plasma_batched.h"
 
 #endif /* PLASMA_H */

@@ -118,6 +118,16 @@ PROTOTYPES.update({
     "plasma_omp_slag2d": (None, [_D, _D, _PS, _PR]),
     "plasma_profile_enable": (c_int, [c_int]),
     "plasma_tuning_load": (c_int, [ctypes.c_char_p]),
+    "plasma_dposv_vbatched": (c_int, [c_int, c_int, c_void_p, c_void_p,
+                                      c_void_p, c_int, c_void_p, c_void_p,
+                                      c_void_p]),
+    "plasma_sposv_vbatched": (c_int, [c_int, c_int, c_void_p, c_void_p,
+                                      c_void_p, c_int, c_void_p, c_void_p,
+                                      c_void_p]),
+    "plasma_dpotrf_vbatched": (c_int, [c_int, c_int, c_void_p, c_void_p,
+                                       c_void_p, c_void_p]),
+    "plasma_spotrf_vbatched": (c_int, [c_int, c_int, c_void_p, c_void_p,
+                                       c_void_p, c_void_p]),
     "plasma_profile_query": (c_int, [c_int, ctypes.POINTER(c_double),
                                      ctypes.POINTER(c_double),
                                      ctypes.POINTER(c_double),

@@ -1,0 +1,237 @@
+/*
+ * batched.c -- SPD solves of many independent systems of uneven order in
+ * one persistent launch (plasma_{d,s}posv_vbatched, plasma_{d,s}potrf_
+ * vbatched).  No counterpart in the reference, which factors one matrix
+ * per call (compute/zposv.c); each system follows the same algorithm as
+ * the reference's tile factorization of a single tile (core_zpotrf.c) and
+ * its triangular solves, with LAPACK's info per system.
+ *
+ * Layout: system i of order n[i] (0 <= n[i] <= 512) is stored column major
+ * at A + aoff[i] with leading dimension n[i]; its right-hand sides at
+ * B + boff[i], n[i] x nrhs, leading dimension n[i].  A and B are device
+ * memory; n, aoff, boff and info may be host or device arrays.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "internal.h"
+
+#define VBATCH_NMAX 512
+
+typedef struct {
+    int n, i;
+} order_t;
+
+static int by_n_desc(const void *a, const void *b)
+{
+    const order_t *x = a, *y = b;
+    if (x->n != y->n)
+        return y->n - x->n;
+    return x->i - y->i;
+}
+
+/* device copy of a host or device array (owned copy in *dev_out) */
+static int stage_in(const void *arr, size_t bytes, void **dev_out,
+                    int *owned)
+{
+    if (plasma_is_device_ptr(arr)) {
+        *dev_out = (void *)arr;
+        *owned = 0;
+        return 0;
+    }
+    *dev_out = plasma_dev_alloc(bytes);
+    if (*dev_out == NULL)
+        return PlasmaErrorOutOfMemory;
+    *owned = 1;
+    return plasma_hip_check(hipMemcpy(*dev_out, arr, bytes,
+                                      hipMemcpyHostToDevice), "hipMemcpy")
+               ? PlasmaErrorInternal : 0;
+}
+
+static int vbatched(plasma_enum_t precision, plasma_enum_t uplo, int batch,
+                    const int *n, void *A, const size_t *aoff, int nrhs,
+                    void *B, const size_t *boff, int *info)
+{
+    plasma_context_t *ctx = plasma_context_self();
+    if (ctx == NULL) {
+        plasma_error("PLASMA not initialized");
+        return PlasmaErrorNotInitialized;
+    }
+    if (uplo != PlasmaUpper && uplo != PlasmaLower) {
+        plasma_error("illegal value of uplo");
+        return -1;
+    }
+    if (batch < 0) {
+        plasma_error("illegal value of batch");
+        return -2;
+    }
+    if (batch == 0)
+        return PlasmaSuccess;
+    if (n == NULL) {
+        plasma_error("NULL n");
+        return -3;
+    }
+    if (A == NULL || !plasma_is_device_ptr(A)) {
+        plasma_error("A must be device memory");
+        return -4;
+    }
+    if (aoff == NULL) {
+        plasma_error("NULL aoff");
+        return -5;
+    }
+    if (nrhs < 0) {
+        plasma_error("illegal value of nrhs");
+        return -6;
+    }
+    if (nrhs > 0 && (B == NULL || !plasma_is_device_ptr(B))) {
+        plasma_error("B must be device memory");
+        return -7;
+    }
+    if (nrhs > 0 && boff == NULL) {
+        plasma_error("NULL boff");
+        return -8;
+    }
+    if (info == NULL) {
+        plasma_error("NULL info");
+        return -9;
+    }
+
+    /* host views of n and aoff: the order and the workspace size */
+    const size_t nbytes = (size_t)batch * sizeof(int);
+    const size_t obytes = (size_t)batch * sizeof(size_t);
+    int *hn = malloc(nbytes);
+    size_t *hoff = malloc(obytes);
+    order_t *ord = malloc((size_t)batch * sizeof(order_t));
+    int *hord = malloc(nbytes);
+    int retval = PlasmaSuccess;
+    void *dn = NULL, *doff = NULL, *dboff = NULL, *dinfo = NULL;
+    int own_n = 0, own_off = 0, own_boff = 0, own_info = 0;
+    void *W = NULL, *dord = NULL, *dticket = NULL;
+    if (hn == NULL || hoff == NULL || ord == NULL || hord == NULL) {
+        retval = PlasmaErrorOutOfMemory;
+        goto done;
+    }
+    if (plasma_hip_check(hipMemcpy(hn, n, nbytes, hipMemcpyDefault),
+                         "hipMemcpy(n)") ||
+        plasma_hip_check(hipMemcpy(hoff, aoff, obytes, hipMemcpyDefault),
+                         "hipMemcpy(aoff)")) {
+        retval = PlasmaErrorInternal;
+        goto done;
+    }
+    size_t wsize = 0;
+    for (int i = 0; i < batch; i++) {
+        if (hn[i] < 0 || hn[i] > VBATCH_NMAX) {
+            plasma_error("illegal order of a system");
+            retval = -3;
+            goto done;
+        }
+        size_t end = hoff[i] + (size_t)hn[i] * hn[i];
+        if (end > wsize)
+            wsize = end;
+        ord[i].n = hn[i];
+        ord[i].i = i;
+    }
+    qsort(ord, batch, sizeof(order_t), by_n_desc);
+    for (int i = 0; i < batch; i++)
+        hord[i] = ord[i].i;
+
+    const size_t es = plasma_element_size(precision);
+    W = plasma_dev_alloc(wsize * es);
+    dord = plasma_dev_alloc(nbytes);
+    dticket = plasma_dev_alloc(sizeof(int));
+    if (W == NULL || dord == NULL || dticket == NULL) {
+        retval = PlasmaErrorOutOfMemory;
+        goto done;
+    }
+    if ((retval = stage_in(n, nbytes, &dn, &own_n)) != 0 ||
+        (retval = stage_in(aoff, obytes, &doff, &own_off)) != 0 ||
+        (nrhs > 0 &&
+         (retval = stage_in(boff, obytes, &dboff, &own_boff)) != 0))
+        goto done;
+    own_info = !plasma_is_device_ptr(info);
+    if (own_info) {
+        dinfo = plasma_dev_alloc(nbytes);
+        if (dinfo == NULL) {
+            retval = PlasmaErrorOutOfMemory;
+            goto done;
+        }
+    }
+    else {
+        dinfo = info;
+    }
+    hipStream_t s = ctx->stream;
+    if (plasma_hip_check(hipMemcpyAsync(dord, hord, nbytes,
+                                        hipMemcpyHostToDevice, s),
+                         "hipMemcpy(order)") ||
+        plasma_kern_fill_int(dticket, 0, 1, s) != 0) {
+        retval = PlasmaErrorInternal;
+        goto done;
+    }
+    /* one resident grid: as many workgroups as the LDS lets the CUs hold */
+    const size_t lds = plasma_kern_vbatched_lds(precision);
+    int per_cu = (int)((160 * 1024) / lds);
+    if (per_cu < 1)
+        per_cu = 1;
+    int grid = ctx->num_cus * per_cu;
+    if (grid > batch)
+        grid = batch;
+    if (plasma_kern_vbatched_posv(precision, uplo == PlasmaUpper, batch, dn,
+                                  A, doff, W, nrhs, B, dboff, dinfo, dord,
+                                  dticket, grid, s) != 0) {
+        retval = PlasmaErrorInternal;
+        goto done;
+    }
+    if (own_info &&
+        plasma_hip_check(hipMemcpyAsync(info, dinfo, nbytes,
+                                        hipMemcpyDeviceToHost, s),
+                         "hipMemcpy(info)")) {
+        retval = PlasmaErrorInternal;
+        goto done;
+    }
+    if (plasma_hip_check(hipStreamSynchronize(s), "hipStreamSynchronize"))
+        retval = PlasmaErrorInternal;
+
+done:
+    if (own_n) plasma_dev_free(dn);
+    if (own_off) plasma_dev_free(doff);
+    if (own_boff) plasma_dev_free(dboff);
+    if (own_info) plasma_dev_free(dinfo);
+    plasma_dev_free(dticket);
+    plasma_dev_free(dord);
+    plasma_dev_free(W);
+    free(hord);
+    free(ord);
+    free(hoff);
+    free(hn);
+    return retval;
+}
+
+int plasma_dposv_vbatched(plasma_enum_t uplo, int batch, const int *n,
+                          double *A, const size_t *aoff, int nrhs, double *B,
+                          const size_t *boff, int *info)
+{
+    return vbatched(PlasmaRealDouble, uplo, batch, n, A, aoff, nrhs, B, boff,
+                    info);
+}
+
+int plasma_sposv_vbatched(plasma_enum_t uplo, int batch, const int *n,
+                          float *A, const size_t *aoff, int nrhs, float *B,
+                          const size_t *boff, int *info)
+{
+    return vbatched(PlasmaRealFloat, uplo, batch, n, A, aoff, nrhs, B, boff,
+                    info);
+}
+
+int plasma_dpotrf_vbatched(plasma_enum_t uplo, int batch, const int *n,
+                           double *A, const size_t *aoff, int *info)
+{
+    return vbatched(PlasmaRealDouble, uplo, batch, n, A, aoff, 0, NULL, NULL,
+                    info);
+}
+
+int plasma_spotrf_vbatched(plasma_enum_t uplo, int batch, const int *n,
+                           float *A, const size_t *aoff, int *info)
+{
+    return vbatched(PlasmaRealFloat, uplo, batch, n, A, aoff, 0, NULL, NULL,
+                    info);
+}

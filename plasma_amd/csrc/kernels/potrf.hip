@@ -256,29 +256,23 @@ __device__ __forceinline__ void inv_block32(const View<T> &L, const View<T> &Wv,
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal tile k: with `factor`, L L^T = A(k,k) by 32-column blocks (a wave
-// factors the 32 x 32 block with readlane, the rows below are multiplied by
-// the block's inverse on MFMA, and the rest of the tile is updated on MFMA);
-// then, always, W = L^{-1} (lower, zeros above) into the workspace tile
-// inv + k * mb * mb (ld = mb), by block columns: W_JJ = L_JJ^{-1},
-// W_IJ = -W_II sum_{K=J}^{I-1} L_IK W_KJ.  The invert-only form runs one
-// workgroup per tile of a batch.
-// A non-positive or NaN pivot at local column j sets the status word to
-// k*nb + j + 1 (reference core_zpotrf.c:72-79, iinfo = A.nb * k).
+// One workgroup factors a tile of order n <= 512: L L^T = A by 32-column
+// blocks (a wave factors and inverts the 32 x 32 block in registers, the
+// rows below are multiplied by the block's inverse on MFMA, and the rest of
+// the tile is updated on MFMA); then W = L^{-1} (lower, zeros above) by
+// block columns: W_JJ = L_JJ^{-1}, W_IJ = -W_II sum_{K=J}^{I-1} L_IK W_KJ.
 // ---------------------------------------------------------------------------
+// mode 0: invert an already factored tile; 1: factor and invert; 2:
+// factor, with only the 32 x 32 diagonal blocks of W inverted.  L: the tile
+// (lower-equivalent view), n its order; Wv: the n x n workspace of W (unit
+// row stride); sP: dynamic LDS of max(n, 8 * JB) x (JB + 1) elements.
+// Returns 0 or the 1-based column of the first failed pivot (the same value
+// in every thread).  All threads of the workgroup must call it.
 template <typename T>
-__global__ __launch_bounds__(NTHREADS) void potrf_diag_kernel(
-    plasma_desc_t A, int upper, int k0, int factor, T *inv, int *status)
+__device__ int factor_tile(const View<T> &L, const int n, const View<T> &Wv,
+                           const int mode, T *sP)
 {
-    if (block_should_stop(status)) return;
-    const int k = k0 + blockIdx.x;
-    const int n = plasma_tile_mview(A, k);
     const int nbb = (n + JB - 1) / JB;
-    const View<T> L = lower_view<T>(A, upper != 0, k, k);
-    View<T> Wv;
-    Wv.p = inv + (size_t)k * A.mb * A.mb;
-    Wv.rs = 1;
-    Wv.cs = A.mb;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     typedef typename Mfma<T>::acc_t acc_t;
 
@@ -287,8 +281,6 @@ __global__ __launch_bounds__(NTHREADS) void potrf_diag_kernel(
     __shared__ T sDblk[JB * LDB];       // the current block, for wave 0
     __shared__ T sB[2 * JB];            // broadcast buffer of wave 0
     __shared__ int s_info;
-    extern __shared__ unsigned char smem_raw[];
-    T *sP = reinterpret_cast<T *>(smem_raw);   // panel rows, n x LDP
     if (t == 0) s_info = 0;
 
     View<T> Dv;                          // Dv(c, l) = Dinv(c, l)
@@ -300,7 +292,7 @@ __global__ __launch_bounds__(NTHREADS) void potrf_diag_kernel(
     Pv.rs = LDP;
     Pv.cs = 1;
 
-    if (factor) {
+    if (mode != 0) {
         for (int J = 0; J < nbb; J++) {
             const int j0 = J * JB, jb = min(JB, n - j0);
             const int i0 = j0 + jb, nrows = n - i0;
@@ -332,10 +324,7 @@ __global__ __launch_bounds__(NTHREADS) void potrf_diag_kernel(
                 }
             }
             __syncthreads();
-            if (s_info) {
-                if (t == 0) atomicCAS(status, 0, A.nb * k + s_info);
-                return;
-            }
+            if (s_info) return s_info;
             // (b) P = A_panel Dinv^T, one 16-row strip per wave (both
             //     16-column halves); P goes back to LDS and to the tile
             for (int st = wave; st * 16 < nrows; st += 4) {
@@ -396,6 +385,8 @@ __global__ __launch_bounds__(NTHREADS) void potrf_diag_kernel(
                            sP + (2 * wave + 1) * JB * LDB, j0, jb);
         }
     }
+
+    if (mode == 2) return 0;
 
     // zero the strict upper triangle of W
     for (int c = wave; c < n; c += 4)
@@ -459,6 +450,31 @@ __global__ __launch_bounds__(NTHREADS) void potrf_diag_kernel(
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal tile k: with `factor`, L L^T = A(k,k) and W = L^{-1}; without,
+// W only (one workgroup per tile of a batch).  A failed pivot at local
+// column j sets the status word to k*nb + j + 1 (reference
+// core_zpotrf.c:72-79, iinfo = A.nb * k).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void potrf_diag_kernel(
+    plasma_desc_t A, int upper, int k0, int factor, T *inv, int *status)
+{
+    if (block_should_stop(status)) return;
+    const int k = k0 + blockIdx.x;
+    View<T> Wv;
+    Wv.p = inv + (size_t)k * A.mb * A.mb;
+    Wv.rs = 1;
+    Wv.cs = A.mb;
+    extern __shared__ unsigned char smem_raw[];
+    T *sP = reinterpret_cast<T *>(smem_raw);
+    const int info = factor_tile<T>(lower_view<T>(A, upper != 0, k, k),
+                                    plasma_tile_mview(A, k), Wv,
+                                    factor ? 1 : 0, sP);
+    if (info && threadIdx.x == 0) atomicCAS(status, 0, A.nb * k + info);
 }
 
 // ---------------------------------------------------------------------------
@@ -665,6 +681,138 @@ int launch_trailing_update(plasma_enum_t uplo, plasma_desc_t A, int pbase,
     return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
 }
 
+// ---------------------------------------------------------------------------
+// Batched SPD solves of uneven order n_i <= 512 (no reference counterpart:
+// the reference factors one matrix at a time).  Each system is one
+// workgroup's work: factor_tile in mode 2 (factor, inverses of the 32 x 32
+// diagonal blocks), then the two triangular solves by 32-row blocks,
+// y_J = W_JJ (b_J - L_J,<J y_<J) and x_J = W_JJ^T (y_J - L_>J,J^T x_>J),
+// with the solution vector in LDS.  A persistent grid takes systems from a
+// ticket counter in the order `order` (largest first); every workgroup
+// exits when the tickets run out.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ void solve_tile(const View<T> &L, const int n, const View<T> &Wv,
+                           T *Bp, const int ldb, const int nrhs, T *sw)
+{
+    const int t = threadIdx.x;
+    const int nbb = (n + JB - 1) / JB;
+    T *sy = sw;                    // the vector being solved, n <= 512
+    T *spart = sw + 512;           // 8 x 32 partial sums
+    T *st = spart + 8 * JB;        // 32
+    for (int c = 0; c < nrhs; c++) {
+        T *b = Bp + (size_t)c * ldb;
+        for (int i = t; i < n; i += NTHREADS) sy[i] = b[i];
+        __syncthreads();
+        for (int J = 0; J < nbb; J++) {            // L y = b
+            const int j0 = J * JB, jb = min(JB, n - j0);
+            const int r = t % JB, p = t / JB;
+            T acc = T(0);
+            if (r < jb)
+                for (int k = p; k < j0; k += NTHREADS / JB)
+                    acc += L.at(j0 + r, k) * sy[k];
+            spart[p * JB + r] = acc;
+            __syncthreads();
+            if (t < jb) {
+                T v = sy[j0 + t];
+                for (int q = 0; q < NTHREADS / JB; q++) v -= spart[q * JB + t];
+                st[t] = v;
+            }
+            __syncthreads();
+            if (t < jb) {
+                T y = T(0);
+                for (int cc = 0; cc <= t; cc++)
+                    y += Wv.at(j0 + t, j0 + cc) * st[cc];
+                sy[j0 + t] = y;
+            }
+            __syncthreads();
+        }
+        for (int J = nbb - 1; J >= 0; J--) {       // L^T x = y
+            const int j0 = J * JB, jb = min(JB, n - j0);
+            const int r = t / (NTHREADS / JB), p = t % (NTHREADS / JB);
+            T acc = T(0);
+            if (r < jb)
+                for (int k = j0 + jb + p; k < n; k += NTHREADS / JB)
+                    acc += L.at(k, j0 + r) * sy[k];
+            spart[p * JB + r] = acc;
+            __syncthreads();
+            if (t < jb) {
+                T v = sy[j0 + t];
+                for (int q = 0; q < NTHREADS / JB; q++) v -= spart[q * JB + t];
+                st[t] = v;
+            }
+            __syncthreads();
+            if (t < jb) {
+                T x = T(0);
+                for (int cc = t; cc < jb; cc++)
+                    x += Wv.at(j0 + cc, j0 + t) * st[cc];
+                sy[j0 + t] = x;
+            }
+            __syncthreads();
+        }
+        for (int i = t; i < n; i += NTHREADS) b[i] = sy[i];
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void vbatched_kernel(
+    int upper, int batch, const int *nvec, T *A, const size_t *aoff, T *W,
+    int nrhs, T *B, const size_t *boff, int *info, const int *order,
+    int *ticket)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T *sP = reinterpret_cast<T *>(smem_raw);
+    __shared__ int s_i;
+    for (;;) {
+        if (threadIdx.x == 0) s_i = atomicAdd(ticket, 1);
+        __syncthreads();
+        const int i = s_i;
+        __syncthreads();
+        if (i >= batch) break;
+        const int sys = order[i];
+        const int n = nvec[sys];
+        int inf = 0;
+        if (n > 0) {
+            View<T> L;
+            L.p = A + aoff[sys];
+            L.rs = upper ? n : 1;
+            L.cs = upper ? 1 : n;
+            View<T> Wv;
+            Wv.p = W + aoff[sys];
+            Wv.rs = 1;
+            Wv.cs = n;
+            inf = factor_tile<T>(L, n, Wv, 2, sP);
+            if (inf == 0 && nrhs > 0)
+                solve_tile<T>(L, n, Wv, B + boff[sys], n, nrhs, sP);
+        }
+        if (threadIdx.x == 0) info[sys] = inf;
+        __syncthreads();
+    }
+}
+
+template <typename T>
+int launch_vbatched(int upper, int batch, const int *n, void *A,
+                    const size_t *aoff, void *W, int nrhs, void *B,
+                    const size_t *boff, int *info, const int *order,
+                    int *ticket, int grid, hipStream_t s)
+{
+    // factor_tile's panel area for n <= 512, and the solve's vectors
+    const size_t lds = (size_t)8 * JB * 2 * (JB + 1) * sizeof(T);
+    static int attr_set[2] = {0, 0};
+    int &done = attr_set[sizeof(T) == 8];
+    if (!done) {
+        hipFuncSetAttribute((const void *)vbatched_kernel<T>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds);
+        done = 1;
+    }
+    hipLaunchKernelGGL(vbatched_kernel<T>, dim3(grid), dim3(NTHREADS), lds,
+                       s, upper, batch, n, (T *)A, aoff, (T *)W, nrhs,
+                       (T *)B, boff, info, order, ticket);
+    return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
+}
+
 }  // namespace
 
 extern "C" int plasma_kern_potrf_diag(plasma_enum_t uplo, plasma_desc_t A,
@@ -721,5 +869,29 @@ extern "C" int plasma_kern_trailing_update(plasma_enum_t uplo,
         return launch_trailing_update<float>(uplo, A, pbase, kdepth, n_first,
                                              n_end, panel, pcols, copy_cols,
                                              max_wgs, status, s);
+    return PlasmaErrorNotSupported;
+}
+
+extern "C" size_t plasma_kern_vbatched_lds(plasma_enum_t precision)
+{
+    const size_t es = precision == PlasmaRealDouble ? 8 : 4;
+    return (size_t)8 * JB * 2 * (JB + 1) * es +
+           (2 * JB * (JB + 1) + 2 * JB) * es + 32;
+}
+
+extern "C" int plasma_kern_vbatched_posv(plasma_enum_t precision, int upper,
+                                         int batch, const int *n, void *A,
+                                         const size_t *aoff, void *W,
+                                         int nrhs, void *B,
+                                         const size_t *boff, int *info,
+                                         const int *order, int *ticket,
+                                         int grid, hipStream_t s)
+{
+    if (precision == PlasmaRealDouble)
+        return launch_vbatched<double>(upper, batch, n, A, aoff, W, nrhs, B,
+                                       boff, info, order, ticket, grid, s);
+    if (precision == PlasmaRealFloat)
+        return launch_vbatched<float>(upper, batch, n, A, aoff, W, nrhs, B,
+                                      boff, info, order, ticket, grid, s);
     return PlasmaErrorNotSupported;
 }

@@ -17,6 +17,12 @@ Matrices (synthetic, generated on the GPU, resident in HBM):
 B is iid N(0,1).  Every solve is checked afterwards with the tester's
 scaled residual ||B - A X||_inf / (n ||A||_inf ||X||_inf eps64) < 10.
 
+Timed steps run with PlasmaGraphCapture enabled (the factorization's
+launch sequence replays from a captured hipGraph) and no per-launch
+profiling; a second, profiled pass of the same solves, launched one by one
+with HIP events around every launch, gives the per-kernel-class times and
+the roofline.  Both times are printed.
+
 One mixed step is one plasma_dsposv call on device-resident LAPACK arrays
 (translation to tiles, FP32 factor, refinement, translation back, final
 synchronisation).  One plain step is the same work as plasma_dposv on
@@ -46,7 +52,7 @@ HBM_PEAK = 8.0e12             # bytes/s
 EPS64 = 2.0 ** -53
 
 PROF_CLASSES = ["trailing_update", "potrf_diag", "trsm_panel", "sweeps",
-                "layout", "mixed"]
+                "layout", "mixed", "adjoint"]
 
 
 def flops_posv(n, nrhs):
@@ -103,6 +109,34 @@ def ggt_spd(n, seed=0, device="cuda"):
     return A
 
 
+def batched_systems(count, seed=0, device="cuda"):
+    """The batched workload: `count` systems of order n_i ~ U{32..512},
+    A_i = exp(-|x - y|^2 / (2 * 0.2^2)) on n_i points ~ U[0,1]^3 plus
+    1e-3 I, one right-hand side ~ N(0,1); packed column major (ld n_i).
+    Returns (n, A, aoff, B, boff) with A, B float64 tensors on the device
+    and n, aoff, boff host numpy arrays."""
+    import numpy as np
+    import torch
+    g = torch.Generator(device=device).manual_seed(seed)
+    rng = np.random.default_rng(seed)
+    n = rng.integers(32, 513, count).astype(np.int32)
+    aoff = np.zeros(count, dtype=np.uint64)
+    boff = np.zeros(count, dtype=np.uint64)
+    aoff[1:] = np.cumsum(n.astype(np.uint64) ** 2)[:-1]
+    boff[1:] = np.cumsum(n.astype(np.uint64))[:-1]
+    A = torch.empty(int((n.astype(np.uint64) ** 2).sum()), dtype=torch.float64,
+                    device=device)
+    B = torch.randn(int(n.sum()), dtype=torch.float64, device=device,
+                    generator=g)
+    for i in range(count):
+        k = int(n[i])
+        x = torch.rand(k, 3, dtype=torch.float64, device=device, generator=g)
+        K = torch.exp(-torch.cdist(x, x).pow_(2) / (2 * 0.2 ** 2))
+        K.diagonal().add_(1e-3)
+        A[int(aoff[i]):int(aoff[i]) + k * k] = K.reshape(-1)
+    return n, A, aoff, B, boff
+
+
 def scaled_residual(A, X, B):
     """||B - A X||_inf / (n ||A||_inf ||X||_inf eps64); X, B column-major
     n x nrhs stored as (nrhs, n) tensors, A symmetric."""
@@ -131,6 +165,10 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=16384,
                     help="order of the CPU-baseline sample (0 skips it)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--batch", type=int, default=4096,
+                    help="systems of the batched line (0 skips it)")
+    ap.add_argument("--adjoint-n", type=int, default=16384,
+                    help="order of the Cholesky-adjoint line (0 skips it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the solves, copy MIB MiB device to device "
                          "once (a kernel of known traffic for scaling "
@@ -293,6 +331,140 @@ def classes(prof, steps):
     return out
 
 
+def batched_line(bench, count, steps):
+    """plasma_{d,s}posv_vbatched on the batched workload: mean time of one
+    launch over `steps` calls (inputs restored between calls, untimed),
+    FP64-equivalent rate sum(n^3/3 + n^2/2 + n/6 + 2 n^2), and the largest
+    scaled residual of 64 sampled systems."""
+    import numpy as np
+    import torch
+    pa, lib = bench.pa, bench.lib
+    n, A0, aoff, B0, boff = batched_systems(count, seed=3)
+    nf = n.astype(np.float64)
+    flops = float((nf ** 3 / 3 + nf ** 2 / 2 + nf / 6 + 2 * nf ** 2).sum())
+    info = np.zeros(count, dtype=np.int32)
+    out = {"workload": "%d SPD systems, n ~ U{32..512}, RBF(0.2) on U[0,1]^3 "
+                       "+ 1e-3 I, one right-hand side each" % count}
+    for prec, dt in (("d", torch.float64), ("s", torch.float32)):
+        # separate working copies (to() returns the input itself when the
+        # type already matches, and the solve overwrites A and B)
+        A = torch.empty(A0.shape, dtype=dt, device=A0.device)
+        B = torch.empty(B0.shape, dtype=dt, device=B0.device)
+        fn = getattr(lib, "plasma_%sposv_vbatched" % prec)
+        ts = []
+        for it in range(steps + 1):
+            A.copy_(A0)
+            B.copy_(B0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rc = fn(pa.PlasmaLower, count, n.ctypes.data, A.data_ptr(),
+                    aoff.ctypes.data, 1, B.data_ptr(), boff.ctypes.data,
+                    info.ctypes.data)
+            ts.append(time.perf_counter() - t0)
+            if rc != 0 or info.any():
+                raise SystemExit("batched %sposv failed: %d" % (prec, rc))
+        t = sum(ts[1:]) / steps
+        worst = 0.0
+        eps = 2.0 ** -53 if prec == "d" else 2.0 ** -24
+        for i in np.linspace(0, count - 1, 64).astype(int):
+            k, ao, bo = int(n[i]), int(aoff[i]), int(boff[i])
+            M = A0[ao:ao + k * k].reshape(k, k)
+            x = B[bo:bo + k].to(torch.float64)
+            r = B0[bo:bo + k] - M @ x
+            res = (r.abs().max() / (k * M.abs().sum(1).max() *
+                                    x.abs().max())).item() / eps
+            worst = max(worst, res)
+        out[{"d": "fp64", "s": "fp32"}[prec]] = {
+            "ms": round(t * 1e3, 3),
+            "tflops": round(flops / t / 1e12, 3),
+            "max_residual_eps": round(worst, 4)}
+        del A, B
+    return out
+
+
+def gp_kernel(n, seed=0, dtype=None):
+    """K = RBF(X, lengthscale 1) + 0.1 I with X (n x 8) iid N(0,1), and
+    y ~ N(0,1): the Gaussian-process log-likelihood workload."""
+    import torch
+    dtype = dtype or torch.float64
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    X = torch.randn(n, 8, dtype=torch.float64, device="cuda", generator=g)
+    sq = (X * X).sum(1)
+    D2 = (sq[:, None] + sq[None, :] - 2.0 * (X @ X.T)).clamp_min(0.0)
+    K = torch.exp(-0.5 * D2)
+    K.diagonal().add_(0.1)
+    y = torch.randn(1, n, dtype=torch.float64, device="cuda", generator=g)
+    return K.to(dtype).contiguous(), y.to(dtype).contiguous()
+
+
+def adjoint_line(bench, n, steps):
+    """Gaussian-process log-likelihood and its gradient dL/dK on the
+    device: forward = potrf + log det + potrs, backward =
+    plasma_?potrf_adjoint (the log-det term, Lbar = -diag(1/L_ii)) +
+    plasma_?potrs_adjoint (the quadratic term, Xbar = -y/2).  Mean times
+    over `steps` calls; the check is max |G K + I/2 - a y^T / 2| (the
+    closed form 1/2 (a a^T - K^{-1}) multiplied by K)."""
+    import torch
+    pa, lib = bench.pa, bench.lib
+    out = {"workload": "GP log-likelihood gradient, K = RBF(X (n x 8) "
+                       "N(0,1), lengthscale 1) + 0.1 I, n = %d, y ~ N(0,1)"
+                       % n}
+    for prec, dt in (("d", torch.float64), ("s", torch.float32)):
+        K, y = gp_kernel(n, seed=5, dtype=dt)
+        F = torch.empty_like(K)
+        a = torch.empty_like(y)
+        G = torch.empty_like(K)
+        xb = torch.empty_like(y)
+        potrf = getattr(lib, "plasma_%spotrf" % prec)
+        potrs = getattr(lib, "plasma_%spotrs" % prec)
+        fadj = getattr(lib, "plasma_%spotrf_adjoint" % prec)
+        sadj = getattr(lib, "plasma_%spotrs_adjoint" % prec)
+        tf, tb = [], []
+        for it in range(steps + 1):
+            F.copy_(K)
+            a.copy_(y)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rc = potrf(pa.PlasmaLower, n, F.data_ptr(), n)
+            rc = rc or potrs(pa.PlasmaLower, n, 1, F.data_ptr(), n,
+                             a.data_ptr(), n)
+            d = F.diagonal()
+            loglik = (-0.5 * (y * a).sum() - d.log().sum()).item()
+            t1 = time.perf_counter()
+            if rc != 0:
+                raise SystemExit("GP forward failed: %d" % rc)
+            G.zero_()
+            G.diagonal().copy_(-1.0 / d)
+            xb.copy_(y).mul_(-0.5)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            rc = fadj(pa.PlasmaLower, n, F.data_ptr(), n, G.data_ptr(), n)
+            rc = rc or sadj(pa.PlasmaLower, n, 1, F.data_ptr(), n,
+                            a.data_ptr(), n, xb.data_ptr(), n, G.data_ptr(),
+                            n)
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
+            if rc != 0:
+                raise SystemExit("GP backward failed: %d" % rc)
+            tf.append(t1 - t0)
+            tb.append(t3 - t2)
+            print("bench: adjoint %s step %d done" % (prec, it),
+                  file=sys.stderr, flush=True)
+        f_ms = sum(tf[1:]) / steps * 1e3
+        b_ms = sum(tb[1:]) / steps * 1e3
+        Gd, Kd = G.double(), K.double()
+        R = Gd @ Kd - 0.5 * (a.double().T @ y.double())
+        R.diagonal().add_(0.5)
+        out[{"d": "fp64", "s": "fp32"}[prec]] = {
+            "forward_ms": round(f_ms, 2), "backward_ms": round(b_ms, 2),
+            "backward_over_forward": round(b_ms / f_ms, 2),
+            "loglik": loglik,
+            "check_max_abs": float(R.abs().max().item())}
+        del K, F, G, Gd, Kd, R
+        torch.cuda.empty_cache()
+    return out
+
+
 def pmc_traffic(name):
     """HBM bytes per launch of a kernel, from the summary of the two
     rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE, each scaled by the same
@@ -339,30 +511,23 @@ def main():
 
     sweep = {}
     head = None
-    graphs = None
     for kappa in kappas:
         A0, _ = householder_spd(n, kappa, seed=0)
         torch.cuda.synchronize()
         tag = "kappa=%.0e" % kappa
-        ms_m, prof_m, res_m, iters = bench.mixed(A0, B0, "mixed " + tag)
-        ms_p, prof_p, res_p = bench.plain(A0, B0, "plain " + tag)
-        if kappa == kappas[-1]:
-            # the same two solves replaying captured factorization graphs
-            # (the profiler is off: it times launches one by one)
-            bench.pa.set_param(bench.pa.PlasmaGraphCapture,
-                               bench.pa.PlasmaEnabled)
-            gm, _, gres_m, _ = bench.mixed(A0, B0, "mixed graphs " + tag,
-                                           profile=False)
-            gp, _, gres_p = bench.plain(A0, B0, "plain graphs " + tag,
-                                        profile=False)
-            bench.pa.set_param(bench.pa.PlasmaGraphCapture,
-                               bench.pa.PlasmaDisabled)
-            graphs = {"mixed_ms": round(gm, 3),
-                      "mixed_tflops": round(fl / (gm * 1e-3) / 1e12, 3),
-                      "mixed_residual_eps": round(gres_m, 4),
-                      "plain_ms": round(gp, 3),
-                      "plain_tflops": round(fl / (gp * 1e-3) / 1e12, 3),
-                      "plain_residual_eps": round(gres_p, 4)}
+        # timed: the library replaying its captured factorization graphs,
+        # no per-launch profiling (the library's fastest mode)
+        bench.pa.set_param(bench.pa.PlasmaGraphCapture,
+                           bench.pa.PlasmaEnabled)
+        ms_m, _, res_m, iters = bench.mixed(A0, B0, "mixed " + tag,
+                                            profile=False)
+        ms_p, _, res_p = bench.plain(A0, B0, "plain " + tag, profile=False)
+        bench.pa.set_param(bench.pa.PlasmaGraphCapture,
+                           bench.pa.PlasmaDisabled)
+        # profiled: the same solves launched one by one, every launch
+        # bracketed by HIP events (kernel classes and the roofline)
+        pms_m, prof_m, _, _ = bench.mixed(A0, B0, "mixed profiled " + tag)
+        pms_p, prof_p, _ = bench.plain(A0, B0, "plain profiled " + tag)
         del A0
         sweep[tag] = {
             "mixed_tflops": round(fl / (ms_m * 1e-3) / 1e12, 3),
@@ -373,8 +538,11 @@ def main():
             "plain_ms": round(ms_p, 3),
             "plain_residual_eps": round(res_p, 4),
             "speedup": round(ms_p / ms_m, 3),
+            "profiled_mixed_ms": round(pms_m, 3),
+            "profiled_plain_ms": round(pms_p, 3),
         }
-        head = (kappa, ms_m, prof_m, res_m, iters, ms_p, prof_p, res_p)
+        head = (kappa, ms_m, prof_m, res_m, iters, ms_p, prof_p, res_p,
+                pms_m, pms_p)
         print("bench: %s %s" % (tag, json.dumps(sweep[tag])),
               file=sys.stderr, flush=True)
 
@@ -382,7 +550,13 @@ def main():
     if not args.no_ggt:
         A0 = ggt_spd(n, seed=0)
         torch.cuda.synchronize()   # the library's streams read A0 next
-        ms_g, prof_g, res_g = bench.plain(A0, B0, "plain G G^T/n + I")
+        bench.pa.set_param(bench.pa.PlasmaGraphCapture,
+                           bench.pa.PlasmaEnabled)
+        ms_g, _, res_g = bench.plain(A0, B0, "plain G G^T/n + I",
+                                     profile=False)
+        bench.pa.set_param(bench.pa.PlasmaGraphCapture,
+                           bench.pa.PlasmaDisabled)
+        _, prof_g, _ = bench.plain(A0, B0, "plain profiled G G^T/n + I")
         del A0
         ggt = {"workload": "plain FP64 dposv (drop-in) n=%d nrhs=%d lower, "
                            "A = G G^T/n + I" % (n, nrhs),
@@ -393,7 +567,8 @@ def main():
                    "syrk+gemm)", FP64_MFMA_PEAK),
                "kernels": classes(prof_g, args.steps)}
 
-    kappa, ms_m, prof_m, res_m, iters, ms_p, prof_p, res_p = head
+    (kappa, ms_m, prof_m, res_m, iters, ms_p, prof_p, res_p, pms_m,
+     pms_p) = head
     line = {
         "metric": "dposv n=32768 time-to-FP64 backward error: mixed-prec vs "
                   "plain FP64; TFLOP/s",
@@ -417,8 +592,13 @@ def main():
         "iterations": iters,
         "residual_eps": round(res_m, 4),
         "backward_error_ok": res_m < 10.0,
+        "timing": "graph mode (captured factorization graphs, no "
+                  "per-launch profiling); roofline and kernels from a "
+                  "profiled eager pass of the same solves right after",
+        "profiled_ms_per_step": round(pms_m, 3),
         "plain_fp64": {"tflops": round(fl / (ms_p * 1e-3) / 1e12, 3),
                        "ms": round(ms_p, 3), "residual_eps": round(res_p, 4),
+                       "profiled_ms": round(pms_p, 3),
                        "roofline": roofline(
                            prof_p, "trailing_update_kernel<double> (FP64 "
                            "MFMA syrk+gemm)", FP64_MFMA_PEAK),
@@ -432,7 +612,10 @@ def main():
                                          "n32768_pmc_traffic")),
         "kernels": classes(prof_m, args.steps),
         "plain_ggt": ggt,
-        "graph_mode": graphs,
+        "batched": batched_line(bench, args.batch, args.steps)
+        if args.batch > 0 else None,
+        "adjoint": adjoint_line(bench, args.adjoint_n, args.steps)
+        if args.adjoint_n > 0 else None,
     }
     if args.calibrate_copy > 0:
         nel = args.calibrate_copy * (1 << 20) // 8

@@ -18,5 +18,6 @@
 #include "plasma_ds.h"
 #include "plasma_mi355x.h"
 #include "plasma_batched.h"
+#include "plasma_adjoint.h"
 
 #endif /* PLASMA_H */

@@ -17,7 +17,8 @@ enum {
     PlasmaProfSweep  = 3,   /* potrs sweeps (diagonal solve + update)     */
     PlasmaProfLayout = 4,   /* LAPACK <-> tile translation                */
     PlasmaProfMixed  = 5,   /* conversions, residual, norms of dsposv     */
-    PlasmaProfClasses = 6
+    PlasmaProfAdjoint = 6,  /* Cholesky / potrs adjoint products          */
+    PlasmaProfClasses = 7
 };
 
 /*

@@ -42,6 +42,7 @@ def load():
             "ora_dsposv": (i, [i, i, i, p, i, p, i, p, i, pi, i]),
             "ora_dlag2s": (i, [i, i, p, i, p, i]),
             "ora_slag2d": (None, [i, i, p, i, p, i]),
+            "ora_dpotrf_adjoint": (i, [i, i, p, i, p, i]),
             "ora_tile_storage_elems": (ctypes.c_long, [i, i, i, i]),
             "ora_lapack2tile": (None, [i, i, i, p, i, p, i]),
             "ora_tile2lapack": (None, [i, i, i, p, p, i, i]),
@@ -107,6 +108,16 @@ def dsposv(uplo, a, b, nb):
                              _ld(a), b.ctypes.data, _ld(b), x.ctypes.data,
                              _ld(x), ctypes.byref(it), nb)
     return x, it.value, info
+
+
+def potrf_adjoint(uplo, l, lbar):
+    """Symmetric gradient Abar (full n x n) from the factor l and the
+    factor's adjoint lbar, both in the uplo triangle."""
+    l = _f(l, np.float64)
+    g = _f(lbar, np.float64).copy(order="F")
+    info = load().ora_dpotrf_adjoint(uplo, l.shape[0], l.ctypes.data, _ld(l),
+                                     g.ctypes.data, _ld(g))
+    return g, info
 
 
 def dlag2s(a):

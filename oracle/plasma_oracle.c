@@ -449,3 +449,74 @@ done:
     free(A); free(B); free(X); free(R); free(As); free(Xs); free(xn); free(rn);
     return info;
 }
+
+/* ---------------------------------------------------------------------
+ * Cholesky adjoint (no reference counterpart; the MI355X library's
+ * plasma_dpotrf_adjoint).  Reverse-mode sweep of the scalar column
+ * Cholesky
+ *     L(j,j) = sqrt(A(j,j) - sum_{k<j} L(j,k)^2)
+ *     L(i,j) = (A(i,j) - sum_{k<j} L(i,k) L(j,k)) / L(j,j),   i > j
+ * in exact reverse order, accumulating Lbar in place (I. Murray,
+ * "Differentiation of the Cholesky decomposition", 2016, the unblocked
+ * reverse sweep).  This gives the gradient with respect to the lower
+ * triangle of A; the symmetric gradient written out (full n x n, both
+ * triangles) halves its off-diagonal entries.  Input: the factor in the
+ * uplo triangle of L (U = L^T for upper) and Lbar in the same triangle
+ * (Ubar = Lbar^T for upper); the other triangles are not read.  Plain
+ * loops in double, O(n^3): small orders only.
+ * --------------------------------------------------------------------- */
+int ora_dpotrf_adjoint(int uplo, int n, const double *pL, int ldl,
+                       double *pLbar, int ldlb)
+{
+    if (n <= 0) return 0;
+    double *L = malloc((size_t)n * n * sizeof(double));
+    double *G = malloc((size_t)n * n * sizeof(double));   /* Lbar, lower */
+    double *Ab = calloc((size_t)n * n, sizeof(double));
+    if (!L || !G || !Ab) { free(L); free(G); free(Ab); return -1; }
+    const int up = (uplo == ORA_UPPER);
+    /* lower-equivalent copies, column-major, (i, j) at i + j n */
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+            int keep = i >= j;
+            size_t src = up ? (size_t)j + (size_t)i * ldl
+                            : (size_t)i + (size_t)j * ldl;
+            size_t srcb = up ? (size_t)j + (size_t)i * ldlb
+                             : (size_t)i + (size_t)j * ldlb;
+            L[i + (size_t)j * n] = keep ? pL[src] : 0.0;
+            G[i + (size_t)j * n] = keep ? pLbar[srcb] : 0.0;
+        }
+#define L_(i, j) L[(i) + (size_t)(j) * n]
+#define G_(i, j) G[(i) + (size_t)(j) * n]
+#define A_(i, j) Ab[(i) + (size_t)(j) * n]
+    for (int j = n - 1; j >= 0; j--) {
+        const double djj = L_(j, j);
+        for (int i = n - 1; i > j; i--) {
+            /* L(i,j) = t / L(j,j) */
+            const double tb = G_(i, j) / djj;
+            G_(j, j) -= G_(i, j) * L_(i, j) / djj;
+            A_(i, j) += tb;
+            for (int k = 0; k < j; k++) {
+                G_(i, k) -= tb * L_(j, k);
+                G_(j, k) -= tb * L_(i, k);
+            }
+        }
+        /* L(j,j) = sqrt(s) */
+        const double sb = G_(j, j) / (2.0 * djj);
+        A_(j, j) += sb;
+        for (int k = 0; k < j; k++) G_(j, k) -= 2.0 * sb * L_(j, k);
+    }
+    /* symmetric gradient: off-diagonal entries split over the pair */
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+            double v = (i == j) ? A_(i, i)
+                     : 0.5 * (i > j ? A_(i, j) : A_(j, i));
+            pLbar[i + (size_t)j * ldlb] = v;
+        }
+#undef L_
+#undef G_
+#undef A_
+    free(L);
+    free(G);
+    free(Ab);
+    return 0;
+}

@@ -56,6 +56,13 @@ int ora_dsposv(int uplo, int n, int nrhs, const double *A, int lda,
                const double *B, int ldb, double *X, int ldx, int *iter,
                int nb);
 
+/* Cholesky adjoint by the scalar reverse-mode sweep (no reference
+ * counterpart): the factor in the uplo triangle of L, Lbar in the uplo
+ * triangle of Lbar (Ubar = Lbar^T for upper); on exit Lbar holds the full
+ * symmetric gradient of f(A) = f(L(A)).  Plain loops, small n only. */
+int ora_dpotrf_adjoint(int uplo, int n, const double *L, int ldl,
+                       double *Lbar, int ldlb);
+
 /* plasma_dlag2s / plasma_slag2d on LAPACK arrays. */
 int ora_dlag2s(int m, int n, const double *A, int lda, float *As, int ldas);
 void ora_slag2d(int m, int n, const float *As, int ldas, double *A, int lda);

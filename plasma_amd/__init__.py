@@ -128,6 +128,16 @@ PROTOTYPES.update({
                                        c_void_p, c_void_p]),
     "plasma_spotrf_vbatched": (c_int, [c_int, c_int, c_void_p, c_void_p,
                                        c_void_p, c_void_p]),
+    "plasma_dpotrf_adjoint": (c_int, [c_int, c_int, c_void_p, c_int,
+                                      c_void_p, c_int]),
+    "plasma_spotrf_adjoint": (c_int, [c_int, c_int, c_void_p, c_int,
+                                      c_void_p, c_int]),
+    "plasma_dpotrs_adjoint": (c_int, [c_int, c_int, c_int, c_void_p, c_int,
+                                      c_void_p, c_int, c_void_p, c_int,
+                                      c_void_p, c_int]),
+    "plasma_spotrs_adjoint": (c_int, [c_int, c_int, c_int, c_void_p, c_int,
+                                      c_void_p, c_int, c_void_p, c_int,
+                                      c_void_p, c_int]),
     "plasma_profile_query": (c_int, [c_int, ctypes.POINTER(c_double),
                                      ctypes.POINTER(c_double),
                                      ctypes.POINTER(c_double),
@@ -199,6 +209,18 @@ def potrs(uplo, n, nrhs, a, lda, b, ldb, prec="d"):
 def posv(uplo, n, nrhs, a, lda, b, ldb, prec="d"):
     return getattr(load(), "plasma_%sposv" % prec)(uplo, n, nrhs, _ptr(a), lda,
                                                     _ptr(b), ldb)
+
+
+def potrf_adjoint(uplo, n, a, lda, abar, ldabar, prec="d"):
+    return getattr(load(), "plasma_%spotrf_adjoint" % prec)(
+        uplo, n, _ptr(a), lda, _ptr(abar), ldabar)
+
+
+def potrs_adjoint(uplo, n, nrhs, a, lda, x, ldx, xbar, ldxbar, abar, ldabar,
+                  prec="d"):
+    return getattr(load(), "plasma_%spotrs_adjoint" % prec)(
+        uplo, n, nrhs, _ptr(a), lda, _ptr(x), ldx, _ptr(xbar), ldxbar,
+        _ptr(abar), ldabar)
 
 
 def dsposv(uplo, n, nrhs, a, lda, b, ldb, x, ldx):

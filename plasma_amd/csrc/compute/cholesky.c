@@ -17,19 +17,7 @@
  */
 #include "internal.h"
 
-/* A caller's LAPACK array: used in place when it is device memory, staged
- * through a device buffer otherwise. */
-typedef struct {
-    void *user;
-    int user_ld;
-    void *dev;
-    int ld;
-    int m, n;
-    size_t es;
-    int staged;
-} lapack_arg_t;
-
-static int arg_stage_in(lapack_arg_t *a, void *p, int ld, int m, int n,
+int plasma_arg_stage_in(lapack_arg_t *a, void *p, int ld, int m, int n,
                         size_t es, int copy_in)
 {
     plasma_context_t *ctx = plasma_context_self();
@@ -60,7 +48,7 @@ static int arg_stage_in(lapack_arg_t *a, void *p, int ld, int m, int n,
     return PlasmaSuccess;
 }
 
-static void arg_stage_out(lapack_arg_t *a, int copy_back)
+void plasma_arg_stage_out(lapack_arg_t *a, int copy_back)
 {
     if (!a->staged)
         return;
@@ -237,7 +225,7 @@ int plasma_xpotrf(plasma_enum_t precision, plasma_enum_t uplo, int n,
     plasma_request_init(&request);
 
     lapack_arg_t a;
-    retval = arg_stage_in(&a, pA, lda, n, n, es, 1);
+    retval = plasma_arg_stage_in(&a, pA, lda, n, n, es, 1);
     if (retval != PlasmaSuccess) {
         plasma_desc_destroy(&A);
         return retval;
@@ -246,7 +234,7 @@ int plasma_xpotrf(plasma_enum_t precision, plasma_enum_t uplo, int n,
     plasma_omp_xpotrf(precision, uplo, A, &sequence, &request);
     plasma_pxdesc2lapack(uplo, A, a.dev, a.ld, &sequence, &request);
     int status = plasma_sequence_sync(&sequence);
-    arg_stage_out(&a, status == PlasmaSuccess);
+    plasma_arg_stage_out(&a, status == PlasmaSuccess);
     plasma_desc_destroy(&A);
     return status;
 }
@@ -304,11 +292,11 @@ static int solve_driver(plasma_enum_t precision, plasma_enum_t uplo, int n,
     plasma_request_init(&request);
 
     lapack_arg_t a, b;
-    retval = arg_stage_in(&a, pA, lda, n, n, es, 1);
+    retval = plasma_arg_stage_in(&a, pA, lda, n, n, es, 1);
     if (retval == PlasmaSuccess) {
-        retval = arg_stage_in(&b, pB, ldb, n, nrhs, es, 1);
+        retval = plasma_arg_stage_in(&b, pB, ldb, n, nrhs, es, 1);
         if (retval != PlasmaSuccess)
-            arg_stage_out(&a, 0);
+            plasma_arg_stage_out(&a, 0);
     }
     if (retval != PlasmaSuccess) {
         plasma_desc_destroy(&A);
@@ -327,8 +315,8 @@ static int solve_driver(plasma_enum_t precision, plasma_enum_t uplo, int n,
     }
     plasma_pxdesc2lapack(PlasmaGeneral, B, b.dev, b.ld, &sequence, &request);
     int status = plasma_sequence_sync(&sequence);
-    arg_stage_out(&a, factor && status == PlasmaSuccess);
-    arg_stage_out(&b, status == PlasmaSuccess);
+    plasma_arg_stage_out(&a, factor && status == PlasmaSuccess);
+    plasma_arg_stage_out(&b, status == PlasmaSuccess);
     plasma_desc_destroy(&A);
     plasma_desc_destroy(&B);
     return status;

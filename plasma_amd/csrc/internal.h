@@ -148,6 +148,23 @@ void  plasma_pool_release_all(plasma_context_t *ctx);
 /* Pointer kind of a caller's LAPACK-layout array. */
 int plasma_is_device_ptr(const void *p);
 
+/* A caller's LAPACK array: used in place when it is device memory, staged
+ * through a device buffer (ld = m) otherwise.  stage_in copies it in on the
+ * context stream when copy_in is set; stage_out copies it back (when staged
+ * and copy_back is set, synchronously) and frees the staging buffer. */
+typedef struct {
+    void *user;
+    int user_ld;
+    void *dev;
+    int ld;
+    int m, n;
+    size_t es;
+    int staged;
+} lapack_arg_t;
+int plasma_arg_stage_in(lapack_arg_t *a, void *p, int ld, int m, int n,
+                        size_t es, int copy_in);
+void plasma_arg_stage_out(lapack_arg_t *a, int copy_back);
+
 /*
  * Parallel tile algorithms (compute/p*.c).  Each enqueues its launches on
  * the context streams and returns; the main stream is the join point.

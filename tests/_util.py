@@ -53,6 +53,16 @@ def ggt_spd(n, seed=0, dtype=np.float64):
     return np.asfortranarray((g @ g.T) / n + np.eye(n)).astype(dtype)
 
 
+def gp_kernel(n, d=8, lengthscale=1.0, noise=0.1, seed=0):
+    """K = RBF(X, lengthscale) + noise I with X (n x d) iid N(0,1): the
+    Gaussian-process log-likelihood workload of the adjoint."""
+    X = np.random.default_rng(seed).standard_normal((n, d))
+    sq = np.sum(X * X, axis=1)
+    D2 = np.maximum(sq[:, None] + sq[None, :] - 2.0 * (X @ X.T), 0.0)
+    K = np.exp(-D2 / (2.0 * lengthscale ** 2)) + noise * np.eye(n)
+    return np.asfortranarray(K)
+
+
 def rhs(n, nrhs, seed=1, dtype=np.float64):
     return np.asfortranarray(
         np.random.default_rng(seed).standard_normal((n, nrhs))).astype(dtype)

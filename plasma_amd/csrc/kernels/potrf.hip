@@ -39,33 +39,61 @@ __device__ __forceinline__ void wave_sync_lds()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Pivot arithmetic.  FP32 uses the hardware square root and reciprocal
+// (1 ulp, as LAPACK's xPOTF2 it scales the column by the reciprocal of the
+// pivot); FP64 keeps the correctly rounded operations.
+__device__ __forceinline__ float pivot_sqrt(float x)
+{
+    return __builtin_amdgcn_sqrtf(x);
+}
+__device__ __forceinline__ double pivot_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float pivot_rcp(float x)
+{
+    return __builtin_amdgcn_rcpf(x);
+}
+__device__ __forceinline__ double pivot_rcp(double x) { return 1.0 / x; }
+
+// Value of x in lane `src` (a compile-time lane after unrolling), read into
+// a scalar register.
+__device__ __forceinline__ float lane_value(float x, int src)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), src));
+}
+__device__ __forceinline__ double lane_value(double x, int src)
+{
+    const long long v = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)v, src);
+    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // Right-looking Cholesky of the block held in d; returns 0 or the 1-based
 // column of the first non-positive (or NaN) pivot (LAPACK dpotf2 rule).
-// sb: 2 * JB elements of LDS.
+// The column loop is straight-line code: the pivot comes from lane j by
+// readlane, every lane updates all of its trailing columns (entries above
+// the diagonal receive values that are never read: the caller keeps the
+// lower triangle only), and a failed pivot is recorded, not returned from
+// inside the loop.  sb: 2 * JB elements of LDS (column j of L broadcast).
 template <typename T>
 __device__ __forceinline__ int wave_potrf32(T (&d)[JB], int jb, T *sb)
 {
     const int r = threadIdx.x & 31;
+    int info = 0;
 #pragma unroll
     for (int j = 0; j < JB; j++) {
-        sb[r] = d[j];                            // column j, unscaled
-        wave_sync_lds();
-        const T djj = sb[j];
-        if (j < jb && !(djj > T(0))) return j + 1;
-        const T ljj = sqrt(djj);
-        const T lrj = (r == j) ? ljj : d[j] / ljj;
-        if (r >= j) d[j] = lrj;
+        const T djj = lane_value(d[j], j);
+        if (info == 0 && j < jb && !(djj > T(0))) info = j + 1;
+        const T ljj = pivot_sqrt(djj);
+        const T rl = pivot_rcp(ljj);
+        const T lrj = (r == j) ? ljj : d[j] * rl;
+        d[j] = lrj;
         sb[JB + r] = lrj;                        // column j of L
         wave_sync_lds();
 #pragma unroll
-        for (int c = j + 1; c < JB; c++) {
-            const T lcj = sb[JB + c];
-            if (r >= c) d[c] -= lrj * lcj;
-        }
-        wave_sync_lds();
-        __builtin_amdgcn_sched_barrier(0);
+        for (int c = j + 1; c < JB; c++) d[c] -= lrj * sb[JB + c];
     }
-    return 0;
+    wave_sync_lds();
+    return info;
 }
 
 // Inverse of the lower-triangular block held in d (rows on lanes): lane l
@@ -81,12 +109,15 @@ __device__ __forceinline__ void wave_trtri32(const T (&d)[JB], T (&inv)[JB],
     wave_sync_lds();
 #pragma unroll
     for (int r = 0; r < JB; r++) {
-        T s = (cl == r) ? T(1) : T(0);
+        // two partial sums halve the dependent chain of the row
+        T s0 = (cl == r) ? T(1) : T(0), s1 = T(0);
 #pragma unroll
-        for (int l = 0; l < r; l++) s -= sl[r * LDB + l] * inv[l];
-        inv[r] = s / sl[r * LDB + r];
-        // keep each row's LDS reads next to their use
-        __builtin_amdgcn_sched_barrier(0);
+        for (int l = 0; l + 1 < r; l += 2) {
+            s0 -= sl[r * LDB + l] * inv[l];
+            s1 -= sl[r * LDB + l + 1] * inv[l + 1];
+        }
+        if (r & 1) s0 -= sl[r * LDB + r - 1] * inv[r - 1];
+        inv[r] = (s0 + s1) * pivot_rcp(sl[r * LDB + r]);
     }
     wave_sync_lds();
 }
@@ -100,21 +131,20 @@ __device__ __forceinline__ void wave_load_block(const View<T> &L, T *sD,
 {
     const int lane = threadIdx.x & 63;
     const int r = lane & 31, h = lane >> 5;
-    for (int c0 = 0; c0 < JB; c0 += 8) {
-        T x[4];
+    // all 16 loads of the lane in flight before the first LDS store
+    T x[JB / 2];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int c = c0 + 2 * u + h;
-            x[u] = L.at(j0 + min(r, jb - 1), j0 + min(c, jb - 1));
-        }
+    for (int u = 0; u < JB / 2; u++) {
+        const int c = 2 * u + h;
+        x[u] = L.at(j0 + min(r, jb - 1), j0 + min(c, jb - 1));
+    }
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int c = c0 + 2 * u + h;
-            T v;
-            if (r < jb && c < jb) v = (c <= r) ? x[u] : T(0);
-            else                  v = (r == c) ? T(1) : T(0);
-            sD[c * LDB + r] = v;
-        }
+    for (int u = 0; u < JB / 2; u++) {
+        const int c = 2 * u + h;
+        T v;
+        if (r < jb && c < jb) v = (c <= r) ? x[u] : T(0);
+        else                  v = (r == c) ? T(1) : T(0);
+        sD[c * LDB + r] = v;
     }
     wave_sync_lds();
 }
@@ -210,15 +240,14 @@ __device__ __forceinline__ View<T> sub(View<T> v, int r, int c)
 }
 
 // Factor and invert the jb x jb diagonal block at (j0, j0) of the tile view
-// L (one wave): the block's L goes back to the tile, its inverse to sW
-// (column major, stride LDB) and to the workspace tile Wv.  sD: LDS block
-// scratch; sb: 2 * JB elements of LDS.  Returns 0 or the block-local 1-based
-// pivot column.
+// L (one wave): the block's L goes to sD (column major, stride LDB) and its
+// inverse to sW (W^T column major, i.e. W row major: W(r, c) =
+// sW[r * LDB + c]); store_diag_blocks writes both to global memory.  sb: 2 *
+// JB elements of LDS.  Returns 0 or the block-local 1-based pivot column.
 template <typename T>
-__device__ __forceinline__ int diag_block32(const View<T> &L, const View<T> &Wv,
-                                            T *sD, T *sW, T *sb, int j0, int jb)
+__device__ __forceinline__ int diag_block32(const View<T> &L, T *sD, T *sW,
+                                            T *sb, int j0, int jb)
 {
-    const int lane = threadIdx.x & 63;
     wave_load_block(L, sD, j0, jb);
     T d[JB], iv[JB];
     wave_rows_from_lds(d, sD);
@@ -228,13 +257,25 @@ __device__ __forceinline__ int diag_block32(const View<T> &L, const View<T> &Wv,
     wave_rows_to_lds(iv, sW);                     // column lane % 32 of W ...
     wave_rows_to_lds(d, sD);                      // ... and L, column major
     wave_sync_lds();
-    // sW holds W^T column major, i.e. W row major: W(r, c) = sW[r * LDB + c]
-    const int r = lane & 31, h = lane >> 5;
-    for (int c = h; c < jb; c += 2) {
-        if (r < jb && r >= c) L.at(j0 + r, j0 + c) = sD[c * LDB + r];
-        if (r < jb) Wv.at(j0 + r, j0 + c) = sW[r * LDB + c];
-    }
     return 0;
+}
+
+// The factored diagonal block (lower triangle of sD) and its inverse (sW)
+// to the tile and the workspace; all threads of the workgroup, after the
+// barrier that follows diag_block32.
+template <typename T>
+__device__ __forceinline__ void store_diag_blocks(const View<T> &L,
+                                                  const View<T> &Wv,
+                                                  const T *sD, const T *sW,
+                                                  int j0, int jb)
+{
+    for (int e = threadIdx.x; e < JB * JB; e += NTHREADS) {
+        const int r = e % JB, c = e / JB;
+        if (r < jb && c < jb) {
+            if (r >= c) L.at(j0 + r, j0 + c) = sD[c * LDB + r];
+            Wv.at(j0 + r, j0 + c) = sW[r * LDB + c];
+        }
+    }
 }
 
 // Invert the factored jb x jb diagonal block at (j0, j0) of L into the
@@ -298,8 +339,8 @@ __device__ int factor_tile(const View<T> &L, const int n, const View<T> &Wv,
             const int i0 = j0 + jb, nrows = n - i0;
             // (a) one wave: factor and invert the diagonal block
             if (wave == 0) {
-                const int info = diag_block32<T>(L, Wv, sDblk, sDinv, sB,
-                                                 j0, jb);
+                const int info = diag_block32<T>(L, sDblk, sDinv, sB, j0,
+                                                 jb);
                 if (info && lane == 0) s_info = j0 + info;
             }
             else {
@@ -325,6 +366,7 @@ __device__ int factor_tile(const View<T> &L, const int n, const View<T> &Wv,
             }
             __syncthreads();
             if (s_info) return s_info;
+            store_diag_blocks<T>(L, Wv, sDblk, sDinv, j0, jb);
             // (b) P = A_panel Dinv^T, one 16-row strip per wave (both
             //     16-column halves); P goes back to LDS and to the tile
             for (int st = wave; st * 16 < nrows; st += 4) {
@@ -348,29 +390,46 @@ __device__ int factor_tile(const View<T> &L, const int n, const View<T> &Wv,
             __syncthreads();
             // (c) trailing part of the tile: A22 -= P P^T, lower 16x16 blocks;
             //     the C values of a block are fetched before its MFMAs
+            //     in groups of UB blocks per wave: the C loads of a group
+            //     are issued together, so a wave waits for one memory
+            //     round trip per group instead of one per block
+            constexpr int UB = 4;
             const int nb16 = (nrows + 15) / 16;
             const int nblk = nb16 * (nb16 + 1) / 2;
-            for (int b = wave; b < nblk; b += 4) {
-                int bi = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
-                while (bi * (bi + 1) / 2 > b) bi--;
-                while ((bi + 1) * (bi + 2) / 2 <= b) bi++;
-                const int bj = b - bi * (bi + 1) / 2;
-                T cv[4];
+            for (int b0 = wave * UB; b0 < nblk; b0 += 4 * UB) {
+                int bi[UB], bj[UB];
+                T cv[UB][4];
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int r = bi * 16 + Mfma<T>::row(lane, q);
-                    const int c = bj * 16 + lane % 16;
-                    const T x = L.at(i0 + min(r, nrows - 1), i0 + min(c, nrows - 1));
-                    cv[q] = (r < nrows && c <= r) ? x : T(0);
+                for (int u = 0; u < UB; u++) {
+                    const int b = min(b0 + u, nblk - 1);
+                    int i = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+                    while (i * (i + 1) / 2 > b) i--;
+                    while ((i + 1) * (i + 2) / 2 <= b) i++;
+                    bi[u] = i;
+                    bj[u] = b - i * (i + 1) / 2;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int r = bi[u] * 16 + Mfma<T>::row(lane, q);
+                        const int c = bj[u] * 16 + lane % 16;
+                        const T x = L.at(i0 + min(r, nrows - 1),
+                                         i0 + min(c, nrows - 1));
+                        cv[u][q] = (r < nrows && c <= r) ? x : T(0);
+                    }
                 }
-                acc_t acc = {0, 0, 0, 0};
-                wave_mma_nt<T>(acc, sub(Pv, bi * 16, 0), nrows - bi * 16,
-                               sub(Pv, bj * 16, 0), nrows - bj * 16, JB);
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int r = bi * 16 + Mfma<T>::row(lane, q);
-                    const int c = bj * 16 + lane % 16;
-                    if (r < nrows && c <= r) L.at(i0 + r, i0 + c) = cv[q] - acc[q];
+                for (int u = 0; u < UB; u++) {
+                    if (b0 + u >= nblk) break;
+                    acc_t acc = {0, 0, 0, 0};
+                    wave_mma_nt<T>(acc, sub(Pv, bi[u] * 16, 0),
+                                   nrows - bi[u] * 16, sub(Pv, bj[u] * 16, 0),
+                                   nrows - bj[u] * 16, JB);
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int r = bi[u] * 16 + Mfma<T>::row(lane, q);
+                        const int c = bj[u] * 16 + lane % 16;
+                        if (r < nrows && c <= r)
+                            L.at(i0 + r, i0 + c) = cv[u][q] - acc[q];
+                    }
                 }
             }
             __syncthreads();
@@ -395,8 +454,10 @@ __device__ int factor_tile(const View<T> &L, const int n, const View<T> &Wv,
     T (*sT)[JB][LDP] = reinterpret_cast<T (*)[JB][LDP]>(sP);  // per wave
 
     // off-diagonal blocks of W; block columns are paired (J, nbb-1-J) per
-    // wave so the longest and shortest dependency chains share a wave
-    for (int slot = wave; slot < nbb; slot += 4) {
+    // wave (slots 2w and 2w+1, then 2w+8 and 2w+9) so the longest and
+    // shortest dependency chains share a wave
+    for (int slot0 = 2 * wave; slot0 < nbb; slot0 += 8)
+    for (int slot = slot0; slot < min(slot0 + 2, nbb); slot++) {
         // slot s -> column s/2 (even s) or nbb - 1 - s/2 (odd s)
         const int J = (slot & 1) ? nbb - 1 - slot / 2 : slot / 2;
         const int j0 = J * JB, jb = min(JB, n - j0);

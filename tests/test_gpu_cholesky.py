@@ -117,6 +117,20 @@ def test_dpotrf_not_pd(plasma, nb_guard, uplo, zerocol):
     assert oinfo == info
 
 
+@pytest.mark.parametrize("uplo", [L, Up])
+@pytest.mark.parametrize("zerocol", [0, 37, 127, 128, 200])
+def test_spotrf_not_pd(plasma, nb_guard, uplo, zerocol):
+    """FP32 tiles of order <= 128 are factored in LDS: LAPACK's info and
+    pA unchanged, as for the global-memory form."""
+    n, nb = 260, 128
+    pa.set_param(pa.PlasmaNb, nb)
+    A = U.tester_spd(n, dtype=np.float32)
+    A[:, zerocol] = 0.0
+    a = A.copy(order="F")
+    assert pa.potrf(uplo, n, a, n, prec="s") == zerocol + 1
+    assert np.array_equal(a, A)
+
+
 def test_argument_codes(plasma):
     a = np.eye(4, order="F")
     b = np.ones((4, 2), order="F")
@@ -135,7 +149,8 @@ def test_argument_codes(plasma):
 
 
 @pytest.mark.parametrize("uplo", [L, Up])
-@pytest.mark.parametrize("n,nb", [(100, 16), (1000, 256), (1536, 512)])
+@pytest.mark.parametrize("n,nb", [(100, 16), (515, 96), (1000, 128),
+                                  (1000, 256), (1536, 512)])
 def test_spotrf_sposv(plasma, nb_guard, uplo, n, nb):
     pa.set_param(pa.PlasmaTuning, pa.PlasmaDisabled)
     pa.set_param(pa.PlasmaNb, nb)

@@ -15,9 +15,11 @@
  * taking the lower triangle with the diagonal halved).  On the device:
  *     Q  = ltl_lower(L, Lbar)           triangular MFMA product, mirrored
  *     G  = L^{-T} Q                     backward sweep, n right-hand sides
- *     H  = L^{-T} G^T  (= L^{-T} Q L^{-1}, symmetric)
- *     Abar = (H + H^T) / 4              exact symmetry
- * The sweeps are the potrs kernels with the diagonal-tile inverses.
+ *     H  = L^{-T} G^T  (= L^{-T} Q L^{-1}, symmetric): lower triangle only,
+ *                                       a third of a full sweep
+ *     Abar = H / 2, mirrored            exact symmetry
+ * The sweeps are the potrs kernels with the diagonal-tile inverses (block
+ * GEMMs for this many right-hand sides).
  *
  * potrs adjoint.  X = A^{-1} B with adjoint Xbar: Bbar = A^{-1} Xbar and
  * Abar -= 1/2 (Bbar X^T + X Bbar^T).
@@ -136,16 +138,17 @@ int plasma_xpotrf_adjoint(plasma_enum_t precision, plasma_enum_t uplo,
         if (!e && sequence.status == PlasmaSuccess) {
             plasma_prof_begin(s, PlasmaProfAdjoint, 0,
                               2.0 * dn * dn * es, &slot);
-            e = issue(plasma_kern_tile_axpby_t(0.0, Q, 1.0, Q, Lb, st, s),
+            e = issue(plasma_kern_tile_axpby_t(0.0, Q, 1.0, Q, Lb, 0, st,
+                                               s),
                       &sequence, &request);
             plasma_prof_end(s, slot);
         }
         if (!e)
-            plasma_pxtrsm_left(uplo, back, A, Lb, inv, &sequence, &request);
+            plasma_pxtrsm_left_lower(uplo, A, Lb, inv, &sequence, &request);
         if (!e && sequence.status == PlasmaSuccess) {
             plasma_prof_begin(s, PlasmaProfAdjoint, 0,
-                              3.0 * dn * dn * es, &slot);
-            issue(plasma_kern_tile_axpby_t(0.25, Lb, 0.25, Lb, Q, st, s),
+                              2.0 * dn * dn * es, &slot);
+            issue(plasma_kern_tile_axpby_t(0.5, Lb, 0.5, Lb, Q, 1, st, s),
                   &sequence, &request);
             plasma_prof_end(s, slot);
         }

@@ -9,8 +9,9 @@
 //    16-deep chunks through double-buffered LDS, MFMA accumulation; the
 //    diagonal tiles of L and Lbar are read with their triangles masked.
 //  * tile_axpby_t_kernel: Z(I,J) = a X(I,J) + b Y(J,I)^T over all tiles of
-//    square general descriptors (transpose, symmetric part), through LDS so
-//    both sides move in contiguous runs.
+//    square general descriptors (transpose, symmetric part), or the
+//    symmetric matrix with lower triangle a X, through LDS so both sides
+//    move in contiguous runs.
 //  * syr2k_lapack_kernel: C -= s (U V^T + V U^T) on LAPACK-layout arrays
 //    (the potrs adjoint's symmetric rank-2k term, nrhs small).
 #include "common.hip.h"
@@ -151,9 +152,12 @@ __global__ __launch_bounds__(NTHREADS) void ltl_lower_kernel(
 }
 
 // Z(I,J) = a X(I,J) + b Y(J,I)^T, one 32 x 32 block per workgroup step.
+// With mirror, element (i, j) (global indices) is a X(i, j) for i >= j and
+// b Y(j, i) for i < j: the symmetric matrix whose lower triangle is a X
+// (a = b, X = Y), reading only the lower triangle.
 template <typename T>
 __global__ __launch_bounds__(NTHREADS) void tile_axpby_t_kernel(
-    T a, plasma_desc_t X, T b, plasma_desc_t Y, plasma_desc_t Z,
+    T a, plasma_desc_t X, T b, plasma_desc_t Y, plasma_desc_t Z, int mirror,
     const int *status)
 {
     __shared__ T sb[32][33];
@@ -163,21 +167,33 @@ __global__ __launch_bounds__(NTHREADS) void tile_axpby_t_kernel(
     const int mv = plasma_tile_mview(Z, I), nv = plasma_tile_nview(Z, J);
     const View<T> Zv = tile_view<T>(Z, I, J);
     const View<T> Yv = tile_view<T>(Y, J, I);      // (c, r) element order
+    const View<T> Xv = tile_view<T>(X, I, J);
+    const bool need_y = !mirror || I <= J;         // some element from Y^T
+    const bool need_x = a != T(0) && (!mirror || I >= J);
     const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;   // 32 x 8
     for (int blk = 0; blk < bpt * bpt; blk++) {
         const int r0 = (blk / bpt) * 32, c0 = (blk % bpt) * 32;
         if (r0 >= mv || c0 >= nv) continue;
-        // Y(J,I) rows c0.., columns r0..: read along its columns
-        for (int cc = ty; cc < 32; cc += 8) {
-            const int yr = c0 + tx, yc = r0 + cc;
-            sb[cc][tx] = (yr < nv && yc < mv) ? Yv.at(yr, yc) : T(0);
+        if (need_y) {
+            // Y(J,I) rows c0.., columns r0..: read along its columns
+            for (int cc = ty; cc < 32; cc += 8) {
+                const int yr = c0 + tx, yc = r0 + cc;
+                sb[cc][tx] = (yr < nv && yc < mv) ? Yv.at(yr, yc) : T(0);
+            }
         }
         __syncthreads();
         for (int cc = ty; cc < 32; cc += 8) {
             const int r = r0 + tx, c = c0 + cc;
             if (r < mv && c < nv) {
-                T v = b * sb[tx][cc];
-                if (a != T(0)) v += a * tile_view<T>(X, I, J).at(r, c);
+                T v;
+                if (mirror) {
+                    const bool low = (I > J) || (I == J && r >= c);
+                    v = low ? a * Xv.at(r, c) : b * sb[tx][cc];
+                }
+                else {
+                    v = b * sb[tx][cc];
+                    if (need_x) v += a * Xv.at(r, c);
+                }
                 Zv.at(r, c) = v;
             }
         }
@@ -226,15 +242,16 @@ extern "C" int plasma_kern_ltl_lower(plasma_enum_t uplo, plasma_desc_t L,
 
 extern "C" int plasma_kern_tile_axpby_t(double a, plasma_desc_t X, double b,
                                         plasma_desc_t Y, plasma_desc_t Z,
-                                        const int *status, hipStream_t s)
+                                        int mirror, const int *status,
+                                        hipStream_t s)
 {
     dim3 grid(Z.mt, Z.nt);
     if (Z.precision == PlasmaRealDouble)
         hipLaunchKernelGGL(tile_axpby_t_kernel<double>, grid, dim3(NTHREADS),
-                           0, s, a, X, b, Y, Z, status);
+                           0, s, a, X, b, Y, Z, mirror, status);
     else if (Z.precision == PlasmaRealFloat)
         hipLaunchKernelGGL(tile_axpby_t_kernel<float>, grid, dim3(NTHREADS), 0,
-                           s, (float)a, X, (float)b, Y, Z, status);
+                           s, (float)a, X, (float)b, Y, Z, mirror, status);
     else
         return PlasmaErrorNotSupported;
     return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;

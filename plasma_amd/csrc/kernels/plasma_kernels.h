@@ -142,14 +142,15 @@ int plasma_kern_vbatched_posv(plasma_enum_t precision, int upper, int batch,
  * equivalent), Lbar lower triangular in a general n x n descriptor (for
  * uplo == PlasmaUpper the descriptor holds Ubar = Lbar^T).  tile_axpby_t:
  * Z(I,J) = a X(I,J) + b Y(J,I)^T over square general descriptors (X unused
- * when a == 0).  syr2k_lapack: C -= scale (U V^T + V U^T) on LAPACK-layout
+ * when a == 0); with mirror, the symmetric matrix whose lower triangle is
+ * a X (elements above the diagonal b Y^T, Y = X, a = b).  syr2k_lapack: C -= scale (U V^T + V U^T) on LAPACK-layout
  * device arrays (U, V: n x k).
  */
 int plasma_kern_ltl_lower(plasma_enum_t uplo, plasma_desc_t L,
                           plasma_desc_t Lbar, plasma_desc_t Q,
                           const int *status, hipStream_t s);
 int plasma_kern_tile_axpby_t(double a, plasma_desc_t X, double b,
-                             plasma_desc_t Y, plasma_desc_t Z,
+                             plasma_desc_t Y, plasma_desc_t Z, int mirror,
                              const int *status, hipStream_t s);
 int plasma_kern_syr2k_lapack(plasma_enum_t precision, int n, int k,
                              double scale, const void *U, int ldu,

@@ -161,11 +161,114 @@ __global__ __launch_bounds__(NTHREADS) void sweep_update_kernel(
     store_rows<G, T>(acc, C, r0, c0, mv, nv, true);
 }
 
+// ---------------------------------------------------------------------------
+// Sweeps with many right-hand sides (B.n >= WIDE_RHS, e.g. the n columns of
+// the Cholesky adjoint): the diagonal product S = op(W_kk) B(k, :) and the
+// update B(m, :) -= op(L) S of one sweep step run as 128 x 128 MFMA block
+// GEMMs (tilegemm.hip.h), one job per (tile row, 128-row block, 128-column
+// block), instead of the 64 x 32 blocks sized for a few dozen columns.
+// ---------------------------------------------------------------------------
+constexpr int WIDE_RHS = 1024;
+
+template <typename T>
+struct SweepDecode {
+    plasma_desc_t A, B;
+    const T *inv;
+    T *S;
+    int k, backward, upper, diag, bpt, cbpt;
+    __device__ bool operator()(int b, tg::Job<T> &j) const
+    {
+        const int ncb = B.nt * cbpt;
+        const int rb = b % bpt, cb = (b / bpt) % ncb, t = b / bpt / ncb;
+        const int tn = cb / cbpt, c0 = (cb % cbpt) * tg::BM;
+        const int r0 = rb * tg::BM;
+        const int nv = plasma_tile_nview(B, tn);
+        const int kv = plasma_tile_mview(B, k);
+        if (c0 >= nv) return false;
+        View<T> Sv;                                  // S(kk, c), tile column tn
+        Sv.p = S + (size_t)tn * B.nb * A.mb;
+        Sv.rs = 1;
+        Sv.cs = A.mb;
+        if (diag) {
+            // S(r0.., c0..) = op(W_kk)(r0.., :) B(k, tn)(:, c0..)
+            if (r0 >= kv) return false;
+            View<T> X;
+            if (!backward) { X.rs = 1; X.cs = A.mb; }
+            else           { X.rs = A.mb; X.cs = 1; }
+            X.p = const_cast<T *>(inv) + (size_t)k * A.mb * A.mb +
+                  (ptrdiff_t)r0 * X.rs;
+            const View<T> Bk = tile_view<T>(B, k, tn);
+            j.X = X;
+            j.Y.p = Bk.p + (ptrdiff_t)c0 * Bk.cs;  // Y(c, kk) = B(k)(kk, c0 + c)
+            j.Y.rs = Bk.cs;
+            j.Y.cs = 1;
+            j.C.p = Sv.p + r0 + (ptrdiff_t)c0 * Sv.cs;
+            j.C.rs = 1;
+            j.C.cs = Sv.cs;
+            j.xrows = kv - r0;
+        }
+        else {
+            // forward: B(m) -= L(m,k) S, m > k; backward: B(m) -= L(k,m)^T S
+            const int m = backward ? t : k + 1 + t;
+            const int mv = plasma_tile_mview(B, m);
+            if (r0 >= mv) return false;
+            View<T> X;
+            if (!backward) {
+                X = lower_view<T>(A, upper != 0, m, k);
+            }
+            else {
+                const View<T> v = lower_view<T>(A, upper != 0, k, m);
+                X.p = v.p;
+                X.rs = v.cs;
+                X.cs = v.rs;
+            }
+            X.p += (ptrdiff_t)r0 * X.rs;
+            const View<T> Bm = tile_view<T>(B, m, tn);
+            j.X = X;
+            j.Y.p = Sv.p + (ptrdiff_t)c0 * Sv.cs;   // Y(c, kk) = S(kk, c0 + c)
+            j.Y.rs = Sv.cs;
+            j.Y.cs = 1;
+            j.C.p = Bm.p + r0 + (ptrdiff_t)c0 * Bm.cs;
+            j.C.rs = 1;
+            j.C.cs = Bm.cs;
+            j.xrows = mv - r0;
+        }
+        j.yrows = nv - c0;
+        j.K = kv;
+        j.lower = false;
+        j.diag_off = 0;
+        return true;
+    }
+};
+
+// One launch of a wide sweep step; ACC/NEG and the X layout (XU: unit row
+// stride) are compile-time in the block GEMM.
+template <typename T, bool ACC, bool XU>
+int launch_wide(const SweepDecode<T> &d, int njobs, const int *status,
+                hipStream_t s)
+{
+    if (njobs <= 0) return 0;
+    hipLaunchKernelGGL((tg::kernel<T, true, ACC, ACC, XU, false,
+                                   SweepDecode<T>>),
+                       dim3(tg::xcd_grid(njobs)), dim3(tg::NT), 0, s, d, njobs,
+                       status);
+    return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
+}
+
 template <typename T>
 int launch_sweep_diag(int backward, plasma_desc_t A, plasma_desc_t B, int k,
                       const void *inv, void *scratch, const int *status,
                       hipStream_t s)
 {
+    if (B.n >= WIDE_RHS) {
+        const int bpt = (A.mb + tg::BM - 1) / tg::BM;
+        const int cbpt = (B.nb + tg::BM - 1) / tg::BM;
+        SweepDecode<T> d{A, B, (const T *)inv, (T *)scratch, k, backward, 0,
+                         1, bpt, cbpt};
+        const int njobs = bpt * B.nt * cbpt;
+        return backward ? launch_wide<T, false, false>(d, njobs, status, s)
+                        : launch_wide<T, false, true>(d, njobs, status, s);
+    }
     const int rchunks = (B.mb + SW_BM - 1) / SW_BM;
     const int cchunks = (B.nb + SW_BN - 1) / SW_BN;
     hipLaunchKernelGGL(sweep_diag_kernel<T>, dim3(rchunks, B.nt * cchunks),
@@ -180,9 +283,25 @@ int launch_sweep_update(plasma_enum_t uplo, int backward, plasma_desc_t A,
                         plasma_desc_t B, int k, const void *scratch,
                         const int *status, hipStream_t s)
 {
-    const int ntiles = backward ? k : B.mt - k - 1;
+    int ntiles = backward ? k : B.mt - k - 1;
     const int rchunks = (B.mb + SW_BM - 1) / SW_BM;
     const int cchunks = (B.nb + SW_BN - 1) / SW_BN;
+    if (B.n >= WIDE_RHS) {
+        // the update as block GEMMs; sweep_update_kernel below then only
+        // copies S into B(k, :)
+        const int upper = (uplo == PlasmaUpper);
+        const int bpt = (A.mb + tg::BM - 1) / tg::BM;
+        const int cbpt = (B.nb + tg::BM - 1) / tg::BM;
+        SweepDecode<T> d{A, B, nullptr, (T *)scratch, k, backward, upper, 0,
+                         bpt, cbpt};
+        const int njobs = bpt * B.nt * cbpt * ntiles;
+        // X has unit row stride for forward-lower and backward-upper
+        const int err = (backward == upper)
+            ? launch_wide<T, true, true>(d, njobs, status, s)
+            : launch_wide<T, true, false>(d, njobs, status, s);
+        if (err) return err;
+        ntiles = 0;
+    }
     hipLaunchKernelGGL(sweep_update_kernel<T>,
                        dim3(ntiles * rchunks + rchunks, B.nt * cchunks),
                        dim3(NTHREADS), 0, s, A, B, uplo == PlasmaUpper,

@@ -118,6 +118,26 @@ def test_dpotrf_not_pd(plasma, nb_guard, uplo, zerocol):
 
 
 @pytest.mark.parametrize("uplo", [L, Up])
+@pytest.mark.parametrize("prec", ["d", "s"])
+def test_potrs_wide_rhs(plasma, nb_guard, uplo, prec):
+    """Sweeps with >= 1024 right-hand sides run as 128 x 128 block GEMMs:
+    edge tiles in both dimensions, both triangles, against the oracle."""
+    n, nb, nrhs = 700, 128, 1100
+    pa.set_param(pa.PlasmaNb, nb)
+    dt = np.float64 if prec == "d" else np.float32
+    A = U.ggt_spd(n, seed=11)
+    F, info = ora.potrf(ora.LOWER if uplo == L else ora.UPPER, A, nb)
+    assert info == 0
+    B = U.rhs(n, nrhs, seed=12)
+    ref, _ = ora.potrs(ora.LOWER if uplo == L else ora.UPPER, F, B, nb)
+    f = np.asfortranarray(F.astype(dt))
+    b = np.asfortranarray(B.astype(dt))
+    assert pa.potrs(uplo, n, nrhs, f, n, b, n, prec=prec) == 0
+    err = np.linalg.norm(b - ref) / np.linalg.norm(ref)
+    assert err < (1e-12 if prec == "d" else 1e-4), err
+
+
+@pytest.mark.parametrize("uplo", [L, Up])
 @pytest.mark.parametrize("zerocol", [0, 37, 127, 128, 200])
 def test_spotrf_not_pd(plasma, nb_guard, uplo, zerocol):
     """FP32 tiles of order <= 128 are factored in LDS: LAPACK's info and

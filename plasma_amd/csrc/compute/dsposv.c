@@ -142,6 +142,9 @@ void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
         return;
     }
     void *inv_d = NULL, *mpan = NULL;
+    /* split-K partials of the residual (optional: NULL runs unsplit) */
+    const size_t symm_bytes = 4 * (size_t)n * nrhs * sizeof(double);
+    void *symm_work = plasma_dev_alloc(symm_bytes);
     double *xnorm = work + lw;
     double *rnorm = xnorm + nrhs;
     double *anorm = rnorm + nrhs;
@@ -168,7 +171,8 @@ void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
 
     /* R = B - A X and the test (reference :466-481) */
     if (issue(plasma_kern_lacpy(B, R, st, s), sequence, request) ||
-        issue(plasma_kern_symm_residual(uplo, A, X, R, st, s), sequence,
+        issue(plasma_kern_symm_residual(uplo, A, X, R, symm_work,
+                                            symm_bytes, st, s), sequence,
               request) ||
         issue(plasma_kern_amax_cols(X, xnorm, st, s), sequence, request) ||
         issue(plasma_kern_amax_cols(R, rnorm, st, s), sequence, request) ||
@@ -188,7 +192,8 @@ void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
             issue(plasma_kern_geadd(1.0, R, 1.0, X, st, s), sequence,
                   request) ||
             issue(plasma_kern_lacpy(B, R, st, s), sequence, request) ||
-            issue(plasma_kern_symm_residual(uplo, A, X, R, st, s), sequence,
+            issue(plasma_kern_symm_residual(uplo, A, X, R, symm_work,
+                                            symm_bytes, st, s), sequence,
                   request) ||
             issue(plasma_kern_amax_cols(X, xnorm, st, s), sequence,
                   request) ||
@@ -220,6 +225,7 @@ void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
     plasma_pxpotrs(uplo, A, X, inv_d, sequence, request);
 
 done:
+    plasma_dev_free(symm_work);
     plasma_dev_free(mpan);
     plasma_dev_free(inv_d);
     plasma_dev_free(inv_s);

@@ -167,7 +167,7 @@ constexpr int SY_BM = 64, SY_BN = 64, SY_KC = 16;
 template <typename T>
 __global__ __launch_bounds__(NTHREADS) void symm_residual_kernel(
     plasma_desc_t A, plasma_desc_t X, plasma_desc_t R, int upper,
-    int rchunks, int cchunks, const int *status)
+    int rchunks, int cchunks, int ksplit, T *part, const int *status)
 {
     // first operand: 64 right-hand sides; second: 64 rows of Asym
     typedef GemmNT<T, SY_BN, SY_BM, SY_KC, 32, 32> G;
@@ -184,6 +184,9 @@ __global__ __launch_bounds__(NTHREADS) void symm_residual_kernel(
     const int t = threadIdx.x;
     const int cpt = (A.mb + SY_KC - 1) / SY_KC;   // chunks per tile column
     const int nchunks = A.mt * cpt;
+    // split-K: workgroup z sums chunks [cbeg, cend) into its own partial
+    const int cbeg = (int)((long)nchunks * blockIdx.z / ksplit);
+    const int cend = (int)((long)nchunks * (blockIdx.z + 1) / ksplit);
     constexpr int W = G::W;
 
     typename G::acc_t acc[G::FM][G::FN];
@@ -242,12 +245,14 @@ __global__ __launch_bounds__(NTHREADS) void symm_residual_kernel(
         G::template store<SY_BM>(rq, sm.y[buf], qu);
     };
 
-    load(0);
-    store(0);
+    if (cbeg < cend) {
+        load(cbeg);
+        store(0);
+    }
     __syncthreads();
-    for (int c = 0; c < nchunks; c++) {
-        const int cur = c & 1;
-        if (c + 1 < nchunks) load(c + 1);
+    for (int c = cbeg; c < cend; c++) {
+        const int cur = (c - cbeg) & 1;
+        if (c + 1 < cend) load(c + 1);
 #pragma unroll
         for (int kk = 0; kk < SY_KC; kk += 4) {
             T a[G::FM], b[G::FN];
@@ -263,10 +268,29 @@ __global__ __launch_bounds__(NTHREADS) void symm_residual_kernel(
                 for (int j = 0; j < G::FN; j++)
                     acc[i][j] = Mfma<T>::mma(a[i], b[j], acc[i][j]);
         }
-        if (c + 1 < nchunks) store(cur ^ 1);
+        if (c + 1 < cend) store(cur ^ 1);
         __syncthreads();
     }
 
+    if (ksplit > 1) {
+        // partial z of element (global row, global column) at
+        // part[(z * R.n + col) * R.m + row]
+        T *pz = part + (size_t)blockIdx.z * R.m * R.n;
+#pragma unroll
+        for (int i = 0; i < G::FM; i++)
+#pragma unroll
+            for (int j = 0; j < G::FN; j++) {
+                const int rr = r0 + G::acc_col(j);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int cc = c0 + G::acc_row(i, q);
+                    if (rr < mv && cc < nv)
+                        pz[(size_t)(tn * R.nb + cc) * R.m + m * R.mb + rr] =
+                            acc[i][j][q];
+                }
+            }
+        return;
+    }
     const View<T> C = tile_view<T>(R, m, tn);
 #pragma unroll
     for (int i = 0; i < G::FM; i++)
@@ -295,16 +319,52 @@ __global__ void conv_test_kernel(const double *rnorm, const double *xnorm,
     *flag = ok;
 }
 
+// R -= sum_z part_z, the partials added in a fixed order (deterministic).
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void symm_reduce_kernel(
+    plasma_desc_t R, int ksplit, const T *part, const int *status)
+{
+    if (block_should_stop(status)) return;
+    const size_t e = (size_t)blockIdx.x * NTHREADS + threadIdx.x;
+    const size_t total = (size_t)R.m * R.n;
+    if (e >= total) return;
+    const int row = (int)(e % R.m), col = (int)(e / R.m);
+    T sum = T(0);
+    for (int z = 0; z < ksplit; z++) sum += part[(size_t)z * total + e];
+    const View<T> C = tile_view<T>(R, row / R.mb, col / R.nb);
+    C.at(row % R.mb, col % R.nb) -= sum;
+}
+
+// The grid has one workgroup per 64 x 64 block of R; when that is small
+// (few right-hand sides) and work has room, K is split so that about
+// SY_TARGET workgroups share the chip.
+constexpr int SY_TARGET = 2048, SY_MAXSPLIT = 8;
+
 template <typename T>
 int launch_symm(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t X,
-                plasma_desc_t R, const int *status, hipStream_t s)
+                plasma_desc_t R, void *work, size_t work_bytes,
+                const int *status, hipStream_t s)
 {
     const int rchunks = (R.mb + SY_BM - 1) / SY_BM;
     const int cchunks = (R.nb + SY_BN - 1) / SY_BN;
+    const long blocks = (long)R.mt * rchunks * R.nt * cchunks;
+    const size_t part_bytes = (size_t)R.m * R.n * sizeof(T);
+    int ksplit = (int)((SY_TARGET + blocks - 1) / blocks);
+    ksplit = min(ksplit, SY_MAXSPLIT);
+    if (work == nullptr || R.i != 0 || R.j != 0) ksplit = 1;
+    else ksplit = (int)min((size_t)ksplit, work_bytes / part_bytes);
+    if (ksplit < 1) ksplit = 1;
     hipLaunchKernelGGL(symm_residual_kernel<T>,
-                       dim3(R.mt * rchunks, R.nt * cchunks), dim3(NTHREADS),
-                       0, s, A, X, R, uplo == PlasmaUpper, rchunks, cchunks,
-                       status);
+                       dim3(R.mt * rchunks, R.nt * cchunks, ksplit),
+                       dim3(NTHREADS), 0, s, A, X, R, uplo == PlasmaUpper,
+                       rchunks, cchunks, ksplit, (T *)work, status);
+    if (ksplit > 1) {
+        const size_t total = (size_t)R.m * R.n;
+        hipLaunchKernelGGL(symm_reduce_kernel<T>,
+                           dim3((unsigned)((total + NTHREADS - 1) / NTHREADS)),
+                           dim3(NTHREADS), 0, s, R, ksplit,
+                           (const T *)work, status);
+    }
     return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
 }
 
@@ -358,14 +418,17 @@ extern "C" int plasma_kern_amax_cols(plasma_desc_t A, double *values,
 
 extern "C" int plasma_kern_symm_residual(plasma_enum_t uplo, plasma_desc_t A,
                                          plasma_desc_t X, plasma_desc_t R,
+                                         void *work, size_t work_bytes,
                                          const int *status, hipStream_t s)
 {
     if (A.precision != X.precision || A.precision != R.precision)
         return PlasmaErrorIllegalValue;
     if (A.precision == PlasmaRealDouble)
-        return launch_symm<double>(uplo, A, X, R, status, s);
+        return launch_symm<double>(uplo, A, X, R, work, work_bytes, status,
+                                   s);
     if (A.precision == PlasmaRealFloat)
-        return launch_symm<float>(uplo, A, X, R, status, s);
+        return launch_symm<float>(uplo, A, X, R, work, work_bytes, status,
+                                  s);
     return PlasmaErrorNotSupported;
 }
 

@@ -181,10 +181,14 @@ int plasma_kern_lansy_inf(plasma_enum_t uplo, plasma_desc_t A, double *work,
 int plasma_kern_amax_cols(plasma_desc_t A, double *values,
                           const int *status, hipStream_t s);
 /* R <- R - A X, A symmetric stored in the uplo triangle of a general
- * descriptor (reference pzhemm.c:44-80 with alpha = -1, beta = 1). */
+ * descriptor (reference pzhemm.c:44-80 with alpha = -1, beta = 1).  work
+ * (device, work_bytes; may be NULL): room for split-K partial sums of R,
+ * used when R has few 64 x 64 blocks; the partials are added in a fixed
+ * order, so results do not depend on scheduling. */
 int plasma_kern_symm_residual(plasma_enum_t uplo, plasma_desc_t A,
-                              plasma_desc_t X, plasma_desc_t R,
-                              const int *status, hipStream_t s);
+                              plasma_desc_t X, plasma_desc_t R, void *work,
+                              size_t work_bytes, const int *status,
+                              hipStream_t s);
 /* *flag = all_j (rnorm[j] <= xnorm[j] * cte), cte = anorm * eps * sqrt(n)
  * (reference zcposv.c:275-287 and :476) */
 int plasma_kern_conv_test(const double *rnorm, const double *xnorm, int nrhs,

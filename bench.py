@@ -155,7 +155,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=32768)
     ap.add_argument("--nrhs", type=int, default=64)
-    ap.add_argument("--nb", type=int, default=256)
+    ap.add_argument("--nb", type=int, default=0,
+                    help="pin the tile size (with --lookahead, update depth "
+                         "2); 0 keeps the library's autotuned schedules")
     ap.add_argument("--lookahead", type=int, default=1)
     ap.add_argument("--kappa", type=str, default="1e2,1e4,1e6",
                     help="condition numbers of the mixed-precision sweep; "
@@ -223,9 +225,31 @@ class Bench:
         rc = lib.plasma_init()
         if rc != 0:
             raise SystemExit("plasma_init failed: %d" % rc)
-        pa.set_param(pa.PlasmaTuning, pa.PlasmaDisabled)
-        pa.set_param(pa.PlasmaNb, args.nb)
-        pa.set_param(pa.PlasmaLookahead, args.lookahead)
+        if args.nb > 0:
+            pa.set_param(pa.PlasmaTuning, pa.PlasmaDisabled)
+            pa.set_param(pa.PlasmaNb, args.nb)
+            pa.set_param(pa.PlasmaLookahead, args.lookahead)
+            pa.set_param(pa.PlasmaUpdateDepth, 2)
+
+    def schedule(self, prec, n):
+        """(nb, update depth, lookahead) of a factorization of order n: the
+        pinned one, or the autotuned table's row the library applies (the
+        first row of the precision with n <= n_max, else the last)."""
+        if self.args.nb > 0:
+            return self.args.nb, 2, self.args.lookahead
+        path = os.environ.get("PLASMA_TUNING_FILE") or os.path.join(
+            ROOT, "plasma_amd", "tuning", "mi355x.txt")
+        rows = []
+        for line in open(path):
+            f = line.split()
+            if len(f) == 6 and f[0] == "potrf" and f[1] == prec:
+                rows.append(tuple(int(x) for x in f[2:]))
+        if not rows:
+            raise SystemExit("no %s rows in %s" % (prec, path))
+        for n_max, nb, depth, la in rows:
+            if n <= n_max:
+                return nb, depth, la
+        return rows[-1][1:]
 
     def prof_reset(self):
         self.lib.plasma_profile_enable(0)
@@ -265,7 +289,12 @@ class Bench:
     def plain(self, A0, B0, label, profile=True):
         """Plain FP64 dposv on device-resident LAPACK arrays."""
         torch, pa, lib = self.torch, self.pa, self.lib
-        n, nrhs, nb = A0.shape[0], B0.shape[0], self.args.nb
+        n, nrhs = A0.shape[0], B0.shape[0]
+        # the tile interface takes nb from the descriptor: apply the
+        # schedule the LAPACK-layout driver would pick
+        nb, depth, la = self.schedule("d", n)
+        pa.set_param(pa.PlasmaUpdateDepth, depth)
+        pa.set_param(pa.PlasmaLookahead, la)
         Aout = torch.empty_like(A0)
         X = torch.empty_like(B0)
         A, B = pa.plasma_desc_t(), pa.plasma_desc_t()
@@ -502,7 +531,7 @@ def main():
         sys.exit(2)
     import torch
     bench = Bench(args)
-    n, nrhs, nb = args.n, args.nrhs, args.nb
+    n, nrhs = args.n, args.nrhs
     fl = flops_posv(n, nrhs)
     kappas = [float(k) for k in args.kappa.split(",") if k]
     g = torch.Generator(device="cuda").manual_seed(1)
@@ -587,8 +616,14 @@ def main():
                                "diag(s) Q^T (64 Householder reflectors), "
                                "cond %.0e; FP32 factor, FP64 refinement"
                                % (n, nrhs, kappa),
-                   "n": n, "nrhs": nrhs, "nb": nb, "uplo": "lower",
-                   "lookahead": args.lookahead, "kappa": kappa},
+                   "n": n, "nrhs": nrhs, "uplo": "lower", "kappa": kappa,
+                   "schedule_fp32_factor": dict(zip(
+                       ("nb", "update_depth", "lookahead"),
+                       bench.schedule("s", n))),
+                   "schedule_fp64_plain": dict(zip(
+                       ("nb", "update_depth", "lookahead"),
+                       bench.schedule("d", n))),
+                   "tuning": "pinned" if args.nb > 0 else "autotuned"},
         "iterations": iters,
         "residual_eps": round(res_m, 4),
         "backward_error_ok": res_m < 10.0,

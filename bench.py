@@ -171,6 +171,9 @@ def parse():
                     help="systems of the batched line (0 skips it)")
     ap.add_argument("--adjoint-n", type=int, default=16384,
                     help="order of the Cholesky-adjoint line (0 skips it)")
+    ap.add_argument("--split-update", choices=("on", "off"), default=None,
+                    help="FP32 trailing updates on bf16 MFMA with split "
+                         "operands (default: the library's setting)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the solves, copy MIB MiB device to device "
                          "once (a kernel of known traffic for scaling "
@@ -225,6 +228,10 @@ class Bench:
         rc = lib.plasma_init()
         if rc != 0:
             raise SystemExit("plasma_init failed: %d" % rc)
+        if args.split_update is not None:
+            pa.set_param(pa.PlasmaSplitUpdate,
+                         pa.PlasmaEnabled if args.split_update == "on"
+                         else pa.PlasmaDisabled)
         if args.nb > 0:
             pa.set_param(pa.PlasmaTuning, pa.PlasmaDisabled)
             pa.set_param(pa.PlasmaNb, args.nb)

@@ -117,6 +117,7 @@ enum {
     PlasmaGraphCapture     = 1002, /* PlasmaEnabled: replay hipGraphs */
     PlasmaDevice           = 1003, /* HIP device ordinal (read only) */
     PlasmaUpdateDepth      = 1004, /* tile columns per trailing update, 1..4 */
+    PlasmaSplitUpdate      = 1005, /* FP32 updates on bf16 MFMA (split terms) */
     PlasmaParamUnknown     = INT_MAX
 };
 

@@ -39,6 +39,7 @@ PlasmaDisabled, PlasmaEnabled = 0, 1
 PlasmaTuning, PlasmaNb, PlasmaIb = 0, 1, 2
 PlasmaLookahead, PlasmaGraphCapture, PlasmaDevice = 1001, 1002, 1003
 PlasmaUpdateDepth = 1004
+PlasmaSplitUpdate = 1005
 
 c_int, c_void_p, c_size_t, c_double = (ctypes.c_int, ctypes.c_void_p,
                                        ctypes.c_size_t, ctypes.c_double)

@@ -89,7 +89,31 @@ typedef struct {
     void *inv;
     int *st;
     int pcols;          /* columns of a panel-buffer tile: g * nb */
+    int split;          /* bf16 split update: planes after the FP32 panel */
 } group_t;
+
+/* FP32 factorizations whose tiles are a multiple of 32 run their trailing
+ * updates on bf16 MFMA with split operands (when enabled) */
+static int use_split(plasma_desc_t A)
+{
+    plasma_context_t *ctx = plasma_context_self();
+    return ctx->split_update == PlasmaEnabled &&
+           A.precision == PlasmaRealFloat && A.nb % 32 == 0 && A.mb == A.nb;
+}
+
+/* Bytes of the FP32 part of a panel buffer, rounded up so that the split
+ * planes that follow it are 256-byte aligned. */
+static size_t panel_fp_bytes(plasma_desc_t A, int pcols)
+{
+    const size_t b = (size_t)imax(A.mt - 1, 1) * A.mb * pcols *
+                     plasma_element_size(A.precision);
+    return (b + 255) / 256 * 256;
+}
+
+static void *planes_of(const group_t *g, const void *panel)
+{
+    return (char *)panel + panel_fp_bytes(g->A, g->pcols);
+}
 
 static int k_panel(const group_t *g, int k, void *panel, int pbase,
                    hipStream_t s)
@@ -99,6 +123,9 @@ static int k_panel(const group_t *g, int k, void *panel, int pbase,
     int err = plasma_kern_trsm_panel(g->uplo, g->A, k, g->inv, panel, pbase,
                                      g->pcols, (k - pbase) * g->A.nb, g->st,
                                      s);
+    if (!err && g->split)
+        err = plasma_kern_split_panel(g->A, k, pbase, g->pcols, panel,
+                                      planes_of(g, panel), g->st, s);
     plasma_prof_end(s, slot);
     return err;
 }
@@ -113,9 +140,14 @@ static int k_update(const group_t *g, int pbase, int ncols, int n0, int n1,
     plasma_prof_begin(s, PlasmaProfUpdate,
                       ctx->profile ? update_flops(g->A, kdepth, n0, n1) : 0,
                       0, &slot);
-    int err = plasma_kern_trailing_update(g->uplo, g->A, pbase, kdepth, n0,
-                                          n1, panel, g->pcols, copy_cols,
-                                          max_wgs, g->st, s);
+    int err = g->split
+        ? plasma_kern_trailing_update_split(g->uplo, g->A, pbase, kdepth, n0,
+                                            n1, panel, planes_of(g, panel),
+                                            g->pcols, copy_cols, max_wgs,
+                                            g->st, s)
+        : plasma_kern_trailing_update(g->uplo, g->A, pbase, kdepth, n0, n1,
+                                      panel, g->pcols, copy_cols, max_wgs,
+                                      g->st, s);
     plasma_prof_end(s, slot);
     return err;
 }
@@ -153,7 +185,7 @@ static int pxpotrf_launch(plasma_enum_t uplo, plasma_desc_t A, void *inv,
     hipStream_t s0 = ctx->stream, s1 = ctx->panel_stream;
     const int mt = A.mt;
     const int gw = imax(1, imin(ctx->group, mt));
-    group_t g = {uplo, A, inv, st, gw * A.nb};
+    group_t g = {uplo, A, inv, st, gw * A.nb, use_split(A)};
 
     if (!lookahead) {
         for (int c0 = 0; c0 < mt; c0 += gw) {
@@ -225,8 +257,9 @@ static int pxpotrf_buffers(plasma_desc_t A, void *inv, int lookahead,
 {
     plasma_context_t *ctx = plasma_context_self();
     const int gw = imax(1, imin(ctx->group, A.mt));
-    const size_t pbytes = (size_t)imax(A.mt - 1, 1) * A.mb * gw * A.nb *
-                          plasma_element_size(A.precision);
+    const size_t pbytes =
+        panel_fp_bytes(A, gw * A.nb) +
+        (use_split(A) ? plasma_split_bytes(A, gw * A.nb) : 0);
     *own_inv = NULL;
     panel[0] = panel[1] = NULL;
     if (inv == NULL && (*own_inv = plasma_dev_alloc(plasma_inv_bytes(A))) ==
@@ -309,7 +342,7 @@ static plasma_graph_entry_t *graph_get(plasma_enum_t uplo, plasma_desc_t A,
         plasma_graph_entry_t *c = &ctx->graphs[i];
         if (c->used && c->uplo == (int)uplo && c->inv_arg == inv &&
             c->lookahead == lookahead && c->group == ctx->group &&
-            memcmp(&c->A, &A, sizeof(A)) == 0)
+            c->split == use_split(A) && memcmp(&c->A, &A, sizeof(A)) == 0)
             e = c;
         if (!c->used || (victim->used && c->stamp < victim->stamp))
             victim = c;
@@ -365,6 +398,7 @@ static plasma_graph_entry_t *graph_get(plasma_enum_t uplo, plasma_desc_t A,
     e->inv_arg = inv;
     e->lookahead = lookahead;
     e->group = ctx->group;
+    e->split = use_split(A);
     e->own_inv = own_inv;
     e->panel[0] = panel[0];
     e->panel[1] = panel[1];

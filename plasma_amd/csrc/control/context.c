@@ -58,6 +58,7 @@ static void context_defaults(plasma_context_t *ctx)
     ctx->householder_mode = PlasmaFlatHouseholder;
     ctx->lookahead = 1;
     ctx->group = 2;
+    ctx->split_update = PlasmaEnabled;
     ctx->graph_capture = PlasmaDisabled;
 }
 
@@ -203,6 +204,13 @@ int plasma_set(plasma_enum_t param, int value)
         }
         ctx->graph_capture = value;
         break;
+    case PlasmaSplitUpdate:
+        if (value != PlasmaEnabled && value != PlasmaDisabled) {
+            plasma_error("invalid split update flag");
+            return PlasmaErrorIllegalValue;
+        }
+        ctx->split_update = value;
+        break;
     default:
         plasma_error("unknown parameter");
         return PlasmaErrorIllegalValue;
@@ -225,6 +233,7 @@ int plasma_get(plasma_enum_t param, int *value)
     case PlasmaGraphCapture:    *value = ctx->graph_capture;     break;
     case PlasmaDevice:          *value = ctx->device;            break;
     case PlasmaUpdateDepth:     *value = ctx->group;             break;
+    case PlasmaSplitUpdate:     *value = ctx->split_update;      break;
     default:
         plasma_error("Unknown parameter");
         return PlasmaErrorIllegalValue;

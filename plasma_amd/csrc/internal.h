@@ -49,7 +49,7 @@ typedef struct {
     plasma_desc_t A;          /* the key: descriptor, uplo, inverse   */
     int uplo;                 /* workspace argument and schedule      */
     const void *inv_arg;
-    int lookahead, group;
+    int lookahead, group, split;
     hipGraphExec_t exec;
     void *own_inv;            /* buffers the graph's launches address */
     void *panel[2];
@@ -76,6 +76,8 @@ struct plasma_context_s {
     /* extensions */
     int lookahead;        /* panel lookahead depth, 0 disables */
     int group;            /* tile columns whose panels update together */
+    int split_update;     /* FP32 trailing updates on bf16 MFMA: panels
+                             split into three bf16 terms, six products */
     int graph_capture;    /* replay cached hipGraphs for the sync drivers */
 
     /* device */

@@ -158,6 +158,25 @@ int plasma_kern_syr2k_lapack(plasma_enum_t precision, int n, int k,
                              hipStream_t s);
 
 /*
+ * FP32 trailing update on bf16 MFMA (no reference counterpart): the panel
+ * columns of step k are split into three bf16 planes per panel tile
+ * (plasma_split_bytes(A, pcols) bytes per panel buffer, row stride
+ * plasma_split_pstride(pcols)); the update then forms each product from six
+ * bf16 products summed in FP32.  kdepth must be a multiple of 32.
+ */
+size_t plasma_split_pstride(int pcols);
+size_t plasma_split_bytes(plasma_desc_t A, int pcols);
+int plasma_kern_split_panel(plasma_desc_t A, int k, int pbase, int pcols,
+                            const void *panel, void *planes,
+                            const int *status, hipStream_t s);
+int plasma_kern_trailing_update_split(plasma_enum_t uplo, plasma_desc_t A,
+                                      int pbase, int kdepth, int n_first,
+                                      int n_end, const void *panel,
+                                      const void *planes, int pcols,
+                                      int copy_cols, int max_wgs,
+                                      const int *status, hipStream_t s);
+
+/*
  * Mixed-precision helpers (reference compute/zcposv.c:363-539).
  */
 /* FP64 -> FP32 (round to nearest), sets *status = 1 on overflow

@@ -34,7 +34,8 @@ def nb_guard(plasma):
     """Restore the tile size / tuning flags a test changes."""
     import plasma_amd as pa
     params = (pa.PlasmaNb, pa.PlasmaTuning, pa.PlasmaLookahead,
-              pa.PlasmaUpdateDepth, pa.PlasmaGraphCapture)
+              pa.PlasmaUpdateDepth, pa.PlasmaGraphCapture,
+              pa.PlasmaSplitUpdate)
     saved = [pa.get_param(p)[1] for p in params]
     # a test that sets nb means that nb: the autotuned table (on by default,
     # as in the reference) is switched off unless the test enables it

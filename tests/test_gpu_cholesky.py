@@ -118,6 +118,27 @@ def test_dpotrf_not_pd(plasma, nb_guard, uplo, zerocol):
 
 
 @pytest.mark.parametrize("uplo", [L, Up])
+@pytest.mark.parametrize("n,nb,depth", [(1000, 128, 2), (2048, 256, 1),
+                                        (1500, 96, 3), (700, 512, 1)])
+def test_spotrf_split_update(plasma, nb_guard, uplo, n, nb, depth):
+    """FP32 trailing updates on bf16 MFMA with operands split into three
+    bf16 terms: the factor matches the FP32-MFMA one to FP32 accuracy
+    (error vs the oracle within 3x of the plain FP32 path's, and <= 1e-5)."""
+    pa.set_param(pa.PlasmaNb, nb)
+    pa.set_param(pa.PlasmaUpdateDepth, depth)
+    A = U.ggt_spd(n, seed=n, dtype=np.float32)
+    ref, _ = ora.potrf(ora.LOWER if uplo == L else ora.UPPER, A, nb,
+                       dtype=np.float32)
+    errs = []
+    for flag in (pa.PlasmaDisabled, pa.PlasmaEnabled):
+        pa.set_param(pa.PlasmaSplitUpdate, flag)
+        a = A.copy(order="F")
+        assert pa.potrf(uplo, n, a, n, prec="s") == 0
+        errs.append(U.factor_error(a, ref, uplo == L))
+    assert errs[1] <= 1e-5 and errs[1] <= 3 * errs[0] + 1e-7, errs
+
+
+@pytest.mark.parametrize("uplo", [L, Up])
 @pytest.mark.parametrize("prec", ["d", "s"])
 def test_potrs_wide_rhs(plasma, nb_guard, uplo, prec):
     """Sweeps with >= 1024 right-hand sides run as 128 x 128 block GEMMs:

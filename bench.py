@@ -48,6 +48,8 @@ sys.path.insert(0, ROOT)
 
 FP64_MFMA_PEAK = 78.6e12      # MI355X dense FP64 matrix peak (FLOP/s)
 FP32_MFMA_PEAK = 157.3e12     # MI355X dense FP32 matrix peak (FLOP/s)
+BF16_MFMA_PEAK = 2.5e15       # MI355X dense bf16 matrix peak (FLOP/s)
+SPLIT_PRODUCTS = 6            # bf16 products per FP32 product (split update)
 HBM_PEAK = 8.0e12             # bytes/s
 EPS64 = 2.0 ** -53
 
@@ -237,6 +239,10 @@ class Bench:
             pa.set_param(pa.PlasmaNb, args.nb)
             pa.set_param(pa.PlasmaLookahead, args.lookahead)
             pa.set_param(pa.PlasmaUpdateDepth, 2)
+
+    def split_on(self):
+        return self.pa.get_param(self.pa.PlasmaSplitUpdate)[1] == \
+            self.pa.PlasmaEnabled
 
     def schedule(self, prec, n):
         """(nb, update depth, lookahead) of a factorization of order n: the
@@ -529,6 +535,25 @@ def roofline(prof, kernel, peak, traffic=None):
             "traffic": traffic}
 
 
+def split_roofline(r, split):
+    """The FP32 update's roofline entry: FP32-algorithmic flops against the
+    FP32 matrix peak; with the split update (bf16 MFMA, six products per
+    FP32 product) also the bf16 MFMA work against the bf16 peak."""
+    if r is None:
+        return None
+    if not split:
+        r["kernel"] += ": tg::kernel<float,...,UpdateDecode> on FP32 MFMA"
+        return r
+    r["kernel"] += (": split_update_kernel, operands split into three bf16 "
+                    "terms, v_mfma_f32_16x16x32_bf16")
+    work = r["achieved"] * 1e12 * SPLIT_PRODUCTS
+    r["mfma_work"] = {"unit": "TFLOP/s", "products_per_fp32_product":
+                      SPLIT_PRODUCTS, "achieved": round(work / 1e12, 1),
+                      "peak": BF16_MFMA_PEAK / 1e12,
+                      "frac": round(work / BF16_MFMA_PEAK, 4)}
+    return r
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -647,11 +672,12 @@ def main():
                        "kernels": classes(prof_p, args.steps)},
         "speedup_vs_plain": round(ms_p / ms_m, 3),
         "kappa_sweep": sweep,
-        "roofline": roofline(prof_m, "trailing_update_kernel<float> (FP32 "
-                             "MFMA syrk+gemm of the FP32 factor)",
-                             FP32_MFMA_PEAK,
-                             pmc_traffic("trailing_update_fp32_dsposv_"
-                                         "n32768_pmc_traffic")),
+        "roofline": split_roofline(
+            roofline(prof_m, "trailing update of the FP32 factor",
+                     FP32_MFMA_PEAK,
+                     pmc_traffic("trailing_update_fp32_dsposv_"
+                                 "n32768_pmc_traffic")),
+            bench.split_on()),
         "kernels": classes(prof_m, args.steps),
         "plain_ggt": ggt,
         "batched": batched_line(bench, args.batch, args.steps)

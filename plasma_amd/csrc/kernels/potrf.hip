@@ -14,6 +14,10 @@
 namespace {
 
 constexpr int NTHREADS = 256;
+// threads of the diagonal-tile factorization: 8 waves in FP32; FP64's
+// block code needs more than the 256 registers per lane 8 waves leave
+template <typename T> struct DiagNT { static constexpr int v = 512; };
+template <> struct DiagNT<double> { static constexpr int v = 256; };
 constexpr int JB = 32;          // column block of the diagonal tile
 
 // ---------------------------------------------------------------------------
@@ -263,13 +267,13 @@ __device__ __forceinline__ int diag_block32(const View<T> &L, T *sD, T *sW,
 // The factored diagonal block (lower triangle of sD) and its inverse (sW)
 // to the tile and the workspace; all threads of the workgroup, after the
 // barrier that follows diag_block32.
-template <typename T>
+template <typename T, int NT>
 __device__ __forceinline__ void store_diag_blocks(const View<T> &L,
                                                   const View<T> &Wv,
                                                   const T *sD, const T *sW,
                                                   int j0, int jb)
 {
-    for (int e = threadIdx.x; e < JB * JB; e += NTHREADS) {
+    for (int e = threadIdx.x; e < JB * JB; e += NT) {
         const int r = e % JB, c = e / JB;
         if (r < jb && c < jb) {
             if (r >= c) L.at(j0 + r, j0 + c) = sD[c * LDB + r];
@@ -309,10 +313,11 @@ __device__ __forceinline__ void inv_block32(const View<T> &L, const View<T> &Wv,
 // row stride); sP: dynamic LDS of max(n, 8 * JB) x (JB + 1) elements.
 // Returns 0 or the 1-based column of the first failed pivot (the same value
 // in every thread).  All threads of the workgroup must call it.
-template <typename T>
+template <typename T, int NT = NTHREADS>
 __device__ int factor_tile(const View<T> &L, const int n, const View<T> &Wv,
                            const int mode, T *sP)
 {
+    constexpr int NW = NT / 64;          // waves of the workgroup
     const int nbb = (n + JB - 1) / JB;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     typedef typename Mfma<T>::acc_t acc_t;
@@ -346,7 +351,7 @@ __device__ int factor_tile(const View<T> &L, const int n, const View<T> &Wv,
             else {
                 // the other waves stage the panel rows below the block,
                 // four loads in flight per thread
-                constexpr int STEP = NTHREADS - 64;
+                constexpr int STEP = NT - 64;
                 const int total = nrows * JB;
                 for (int e0 = t - 64; e0 < total; e0 += 4 * STEP) {
                     T x[4];
@@ -366,10 +371,10 @@ __device__ int factor_tile(const View<T> &L, const int n, const View<T> &Wv,
             }
             __syncthreads();
             if (s_info) return s_info;
-            store_diag_blocks<T>(L, Wv, sDblk, sDinv, j0, jb);
+            store_diag_blocks<T, NT>(L, Wv, sDblk, sDinv, j0, jb);
             // (b) P = A_panel Dinv^T, one 16-row strip per wave (both
             //     16-column halves); P goes back to LDS and to the tile
-            for (int st = wave; st * 16 < nrows; st += 4) {
+            for (int st = wave; st * 16 < nrows; st += NW) {
                 acc_t a0 = {0, 0, 0, 0}, a1 = {0, 0, 0, 0};
                 const View<T> P = sub(Pv, st * 16, 0);
                 const int pr = min(16, nrows - st * 16);
@@ -396,7 +401,7 @@ __device__ int factor_tile(const View<T> &L, const int n, const View<T> &Wv,
             constexpr int UB = 4;
             const int nb16 = (nrows + 15) / 16;
             const int nblk = nb16 * (nb16 + 1) / 2;
-            for (int b0 = wave * UB; b0 < nblk; b0 += 4 * UB) {
+            for (int b0 = wave * UB; b0 < nblk; b0 += NW * UB) {
                 int bi[UB], bj[UB];
                 T cv[UB][4];
 #pragma unroll
@@ -437,7 +442,7 @@ __device__ int factor_tile(const View<T> &L, const int n, const View<T> &Wv,
     }
     else {
         // invert-only: the diagonal blocks' inverses first
-        for (int J = wave; J < nbb; J += 4) {
+        for (int J = wave; J < nbb; J += NW) {
             const int j0 = J * JB, jb = min(JB, n - j0);
             // per-wave block scratch in the (still unused) panel area
             inv_block32<T>(L, Wv, sP + 2 * wave * JB * LDB,
@@ -448,7 +453,7 @@ __device__ int factor_tile(const View<T> &L, const int n, const View<T> &Wv,
     if (mode == 2) return 0;
 
     // zero the strict upper triangle of W
-    for (int c = wave; c < n; c += 4)
+    for (int c = wave; c < n; c += NW)
         for (int r = lane; r < c; r += 64) Wv.at(r, c) = T(0);
     __syncthreads();
     T (*sT)[JB][LDP] = reinterpret_cast<T (*)[JB][LDP]>(sP);  // per wave
@@ -456,7 +461,7 @@ __device__ int factor_tile(const View<T> &L, const int n, const View<T> &Wv,
     // off-diagonal blocks of W; block columns are paired (J, nbb-1-J) per
     // wave (slots 2w and 2w+1, then 2w+8 and 2w+9) so the longest and
     // shortest dependency chains share a wave
-    for (int slot0 = 2 * wave; slot0 < nbb; slot0 += 8)
+    for (int slot0 = 2 * wave; slot0 < nbb; slot0 += 2 * NW)
     for (int slot = slot0; slot < min(slot0 + 2, nbb); slot++) {
         // slot s -> column s/2 (even s) or nbb - 1 - s/2 (odd s)
         const int J = (slot & 1) ? nbb - 1 - slot / 2 : slot / 2;
@@ -520,8 +525,8 @@ __device__ int factor_tile(const View<T> &L, const int n, const View<T> &Wv,
 // column j sets the status word to k*nb + j + 1 (reference
 // core_zpotrf.c:72-79, iinfo = A.nb * k).
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(NTHREADS) void potrf_diag_kernel(
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void potrf_diag_kernel(
     plasma_desc_t A, int upper, int k0, int factor, T *inv, int *status)
 {
     if (block_should_stop(status)) return;
@@ -532,9 +537,9 @@ __global__ __launch_bounds__(NTHREADS) void potrf_diag_kernel(
     Wv.cs = A.mb;
     extern __shared__ unsigned char smem_raw[];
     T *sP = reinterpret_cast<T *>(smem_raw);
-    const int info = factor_tile<T>(lower_view<T>(A, upper != 0, k, k),
-                                    plasma_tile_mview(A, k), Wv,
-                                    factor ? 1 : 0, sP);
+    const int info = factor_tile<T, NT>(lower_view<T>(A, upper != 0, k, k),
+                                        plasma_tile_mview(A, k), Wv,
+                                        factor ? 1 : 0, sP);
     if (info && threadIdx.x == 0) atomicCAS(status, 0, A.nb * k + info);
 }
 
@@ -659,24 +664,32 @@ int launch_potrf_diag(plasma_enum_t uplo, plasma_desc_t A, int k0, int count,
 {
     if (count <= 0) return 0;
     // panel rows (mb x 33) in the factor phase; 8 blocks of 32 x 33 for the
-    // invert-only form and 4 in the inverse; the static part is sDinv,
-    // sDblk, sB and the flags
+    // invert-only form, one per wave in the inverse (8 with the 512-thread
+    // factorization); the static part is sDinv, sDblk, sB and the flags
     const size_t lds = (size_t)max(A.mb, 8 * JB) * (JB + 1) * sizeof(T);
     const size_t lds_static = (2 * JB * (JB + 1) + 2 * JB) * sizeof(T) + 16;
     if (lds + lds_static > 160 * 1024) return PlasmaErrorNotSupported;
+    // the factorization (one tile, on the critical path) runs 8 waves; the
+    // invert-only form (a batch of tiles) 4 per tile
+    const void *fn = factor ? (const void *)potrf_diag_kernel<T, DiagNT<T>::v>
+                            : (const void *)potrf_diag_kernel<T, NTHREADS>;
     if (lds > 65536) {
-        static int attr_set[2] = {0, 0};
-        int &done = attr_set[sizeof(T) == 8];
+        static int attr_set[2][2] = {{0, 0}, {0, 0}};
+        int &done = attr_set[sizeof(T) == 8][factor != 0];
         if (!done) {
-            hipFuncSetAttribute((const void *)potrf_diag_kernel<T>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
+            hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds);
             done = 1;
         }
     }
-    hipLaunchKernelGGL(potrf_diag_kernel<T>, dim3(count), dim3(NTHREADS), lds,
-                       s, A, uplo == PlasmaUpper, k0, factor, (T *)inv,
-                       status);
+    if (factor)
+        hipLaunchKernelGGL((potrf_diag_kernel<T, DiagNT<T>::v>), dim3(count),
+                           dim3(DiagNT<T>::v), lds, s, A, uplo == PlasmaUpper, k0,
+                           factor, (T *)inv, status);
+    else
+        hipLaunchKernelGGL((potrf_diag_kernel<T, NTHREADS>), dim3(count),
+                           dim3(NTHREADS), lds, s, A, uplo == PlasmaUpper, k0,
+                           factor, (T *)inv, status);
     return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
 }
 

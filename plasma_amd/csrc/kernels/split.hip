@@ -99,9 +99,43 @@ __global__ __launch_bounds__(NT) void split_panel_kernel(
 // ---------------------------------------------------------------------------
 // Trailing update on the planes.
 // ---------------------------------------------------------------------------
+// Tile t of the trapezoid {(m, n): n in [n_first, n_first + ncols),
+// m >= n} in pair order: columns are taken two at a time, and within a
+// pair (c, c + 1) the tiles go (c, c), (c+1, c), (c+1, c+1), (c+2, c),
+// (c+2, c+1), ...  so a panel row block X(m) serves both columns back to
+// back while the two Y blocks stay in the XCD's L2.  With T = mt - n_first,
+// pair p holds 2 T - 4 p - 1 tiles (a last, unpaired column T - 2 p), and
+// P(p) = p (2 T + 1 - 2 p) tiles precede it.
+__device__ __forceinline__ void pair_decode(int t, int mt, int n_first,
+                                            int ncols, int *m, int *n)
+{
+    const long T = mt - n_first;
+    const double b = 2.0 * T + 1.0;
+    long p = (long)((b - sqrt(b * b - 8.0 * (double)t)) * 0.25);
+    if (p < 0) p = 0;
+    auto pre = [&](long q) { return q * (2 * T + 1 - 2 * q); };
+    while (p > 0 && pre(p) > t) p--;
+    while (2 * (p + 1) < ncols && pre(p + 1) <= t) p++;
+    const int c = n_first + 2 * (int)p;
+    const int idx = (int)(t - pre(p));
+    if (2 * p + 1 >= ncols) {                    // unpaired last column
+        *n = c;
+        *m = c + idx;
+    }
+    else if (idx == 0) {
+        *n = c;
+        *m = c;
+    }
+    else {
+        *m = c + 1 + (idx - 1) / 2;
+        *n = c + (idx - 1) % 2;
+    }
+}
+
 struct SplitUpdate {
     plasma_desc_t A;
     int pbase, pcols, kdepth, ncols, n_first, bpt, upper, ncopy, pstride;
+    int ucols;                       // updated tile columns from n_first
     const float *panel;
     const __bf16 *planes;
 };
@@ -284,7 +318,7 @@ __global__ __launch_bounds__(NT, 2) void split_update_kernel(
         const int sub_ = jb % per_tile;
         const int bi = sub_ / u.bpt, bj = sub_ % u.bpt;
         int m, n;
-        trapezoid_decode(jb / per_tile, u.A.mt, u.n_first, &m, &n);
+        pair_decode(jb / per_tile, u.A.mt, u.n_first, u.ucols, &m, &n);
         if (m == n && bj > bi) continue;
         const int mv = plasma_tile_mview(u.A, m), nv = plasma_tile_mview(u.A, n);
         const int r0 = bi * BM, c0 = bj * BM;
@@ -344,8 +378,8 @@ extern "C" int plasma_kern_trailing_update_split(
     if (njobs > 0x7fffffffL) return PlasmaErrorNotSupported;
     SplitUpdate u{A, pbase, pcols, kdepth, copy_cols > 0 ? copy_cols : 1,
                   n_first, bpt, uplo == PlasmaUpper, ncopy,
-                  (int)plasma_split_pstride(pcols), (const float *)panel,
-                  (const __bf16 *)planes};
+                  (int)plasma_split_pstride(pcols), max(n_end - n_first, 0),
+                  (const float *)panel, (const __bf16 *)planes};
     const long padded = tg::xcd_grid((int)njobs);
     const long grid = max_wgs > 0 ? min(padded, (long)max_wgs) : padded;
     if (uplo == PlasmaUpper)

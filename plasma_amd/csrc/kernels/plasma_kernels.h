@@ -50,6 +50,20 @@ int plasma_kern_tile2lapack(plasma_enum_t region, plasma_desc_t A, void *pA,
 int plasma_kern_potrf_diag(plasma_enum_t uplo, plasma_desc_t A, int k,
                            void *inv, int *status, hipStream_t s);
 
+/* Diagonal tile k factored with only the inverses of its 32 x 32 diagonal
+ * blocks written to W_kk (the rest of W_kk is left as it was), for the
+ * blocked panel solve below. */
+int plasma_kern_potrf_diag_blocks(plasma_enum_t uplo, plasma_desc_t A, int k,
+                                  void *inv, int *status, hipStream_t s);
+/* Panel solve P(m) = A(m,k) L_kk^{-T} by 32-column block substitution with
+ * the diagonal-block inverses; same output as plasma_kern_trsm_panel.
+ * plasma_panel_blocked_ok: whether its LDS fits A's tile size. */
+int plasma_panel_blocked_ok(plasma_desc_t A);
+int plasma_kern_trsm_panel_blocked(plasma_enum_t uplo, plasma_desc_t A, int k,
+                                   const void *inv, void *panel, int pbase,
+                                   int pcols, int pcol, const int *status,
+                                   hipStream_t s);
+
 /* W_kk = L(k,k)^{-1} for every diagonal tile of an already factored A, one
  * workgroup per tile (used by potrs without a preceding factorization). */
 int plasma_kern_trtri_diag(plasma_enum_t uplo, plasma_desc_t A, void *inv,

@@ -538,8 +538,8 @@ __global__ __launch_bounds__(NT) void potrf_diag_kernel(
     extern __shared__ unsigned char smem_raw[];
     T *sP = reinterpret_cast<T *>(smem_raw);
     const int info = factor_tile<T, NT>(lower_view<T>(A, upper != 0, k, k),
-                                        plasma_tile_mview(A, k), Wv,
-                                        factor ? 1 : 0, sP);
+                                        plasma_tile_mview(A, k), Wv, factor,
+                                        sP);
     if (info && threadIdx.x == 0) atomicCAS(status, 0, A.nb * k + info);
 }
 
@@ -690,6 +690,131 @@ int launch_potrf_diag(plasma_enum_t uplo, plasma_desc_t A, int k0, int count,
         hipLaunchKernelGGL((potrf_diag_kernel<T, NTHREADS>), dim3(count),
                            dim3(NTHREADS), lds, s, A, uplo == PlasmaUpper, k0,
                            factor, (T *)inv, status);
+    return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
+}
+
+// ---------------------------------------------------------------------------
+// Blocked panel solve: P(m) = A(m,k) L_kk^{-T} from the 32 x 32 diagonal-block
+// inverses D_J = L_JJ^{-1} alone (factor_tile mode 2), so the diagonal
+// kernel need not form the whole W_kk on the critical path.  One workgroup
+// per 32-row block of a panel tile; with X the solved rows,
+//     T   = A_J - X(:, 0:j0) L_kk(J, 0:j0)^T        (block column J)
+//     X_J = T D_J^T
+// for J = 0, 1, ...  The row block of A (becoming X) and the block row of
+// L_kk are staged in LDS (row major, k contiguous); each of the 4 waves
+// computes one 16 x 16 quarter of T and of X_J on MFMA.
+// ---------------------------------------------------------------------------
+constexpr int PB_RB = 32;           // rows per workgroup
+
+__host__ __device__ constexpr int pb_ld(int nb) { return nb + 4; }
+
+template <typename T>
+size_t pb_lds_bytes(int nb)
+{
+    return (2 * (size_t)PB_RB * pb_ld(nb) + 2 * JB * (JB + 1)) * sizeof(T);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void panel_blocked_kernel(
+    plasma_desc_t A, int upper, int k, const T *inv, T *panel, int pbase,
+    int pcols, int pcol, int nrb, const int *status)
+{
+    if (block_should_stop(status)) return;
+    const int m = k + 1 + blockIdx.x / nrb;
+    const int r0 = (blockIdx.x % nrb) * PB_RB;
+    const int mv = plasma_tile_mview(A, m), kv = plasma_tile_mview(A, k);
+    if (r0 >= mv) return;
+    const int rows = min(PB_RB, mv - r0);
+    const int LDX = pb_ld(A.nb);
+    extern __shared__ unsigned char smem_raw[];
+    T *sX = reinterpret_cast<T *>(smem_raw);           // PB_RB x LDX
+    T *sL = sX + PB_RB * LDX;                           // JB x LDX
+    T *sT = sL + JB * LDX;                              // JB x (JB + 1)
+    T *sD = sT + JB * (JB + 1);                         // JB x (JB + 1)
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const View<T> Amk = lower_view<T>(A, upper != 0, m, k);
+    const View<T> Lkk = lower_view<T>(A, upper != 0, k, k);
+    const T *W = inv + (size_t)k * A.mb * A.mb;         // column major, ld mb
+    T *P = panel + (size_t)(m - pbase - 1) * A.mb * pcols +
+           (size_t)pcol * A.mb;                          // column major, ld mb
+
+    // the row block of A(m,k); rows past the tile read as zero
+    for (int e = t; e < PB_RB * kv; e += NTHREADS) {
+        const int r = e % PB_RB, c = e / PB_RB;
+        sX[r * LDX + c] = (r < rows) ? Amk.at(r0 + r, c) : T(0);
+    }
+    const int wr = wave >> 1, wc = wave & 1;             // 16 x 16 quarter
+    typedef typename Mfma<T>::acc_t acc_t;
+    for (int j0 = 0; j0 < kv; j0 += JB) {
+        const int jb = min(JB, kv - j0);
+        // block row J of L_kk left of the diagonal, and D_J (zero padded)
+        for (int e = t; e < JB * j0; e += NTHREADS) {
+            const int c = e % JB, kk = e / JB;
+            sL[c * LDX + kk] = (c < jb) ? Lkk.at(j0 + c, kk) : T(0);
+        }
+        for (int e = t; e < JB * JB; e += NTHREADS) {
+            const int c = e % JB, l = e / JB;
+            sD[c * (JB + 1) + l] = (c < jb && l < jb)
+                ? W[(size_t)(j0 + l) * A.mb + j0 + c] : T(0);
+        }
+        __syncthreads();
+        // T = A_J - X(:, 0:j0) L(J, 0:j0)^T
+        acc_t acc = {0, 0, 0, 0};
+        for (int kk = 0; kk < j0; kk += 4) {
+            const T a = sX[(wr * 16 + lane % 16) * LDX + kk + lane / 16];
+            const T b = sL[(wc * 16 + lane % 16) * LDX + kk + lane / 16];
+            acc = Mfma<T>::mma(a, b, acc);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int r = wr * 16 + Mfma<T>::row(lane, q);
+            const int c = wc * 16 + lane % 16;
+            const T aj = (c < jb) ? sX[r * LDX + j0 + c] : T(0);
+            sT[r * (JB + 1) + c] = aj - acc[q];
+        }
+        __syncthreads();
+        // X_J = T D_J^T
+        acc_t x = {0, 0, 0, 0};
+#pragma unroll
+        for (int l = 0; l < JB; l += 4) {
+            const T a = sT[(wr * 16 + lane % 16) * (JB + 1) + l + lane / 16];
+            const T b = sD[(wc * 16 + lane % 16) * (JB + 1) + l + lane / 16];
+            x = Mfma<T>::mma(a, b, x);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int r = wr * 16 + Mfma<T>::row(lane, q);
+            const int c = wc * 16 + lane % 16;
+            if (c < jb) {
+                sX[r * LDX + j0 + c] = x[q];
+                if (r < rows) P[(size_t)(j0 + c) * A.mb + r0 + r] = x[q];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+int launch_panel_blocked(plasma_enum_t uplo, plasma_desc_t A, int k,
+                         const void *inv, void *panel, int pbase, int pcols,
+                         int pcol, const int *status, hipStream_t s)
+{
+    const int ntiles = A.mt - k - 1;
+    if (ntiles <= 0) return 0;
+    const size_t lds = pb_lds_bytes<T>(A.nb);
+    if (lds > 160 * 1024) return PlasmaErrorNotSupported;
+    static int attr_done = 0;
+    if (!attr_done && lds > 65536) {
+        hipFuncSetAttribute((const void *)panel_blocked_kernel<T>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds);
+        attr_done = 1;
+    }
+    const int nrb = (A.mb + PB_RB - 1) / PB_RB;
+    hipLaunchKernelGGL(panel_blocked_kernel<T>, dim3(ntiles * nrb),
+                       dim3(NTHREADS), lds, s, A, uplo == PlasmaUpper, k,
+                       (const T *)inv, (T *)panel, pbase, pcols, pcol, nrb,
+                       status);
     return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
 }
 
@@ -897,6 +1022,44 @@ extern "C" int plasma_kern_potrf_diag(plasma_enum_t uplo, plasma_desc_t A,
         return launch_potrf_diag<double>(uplo, A, k, 1, 1, inv, status, s);
     if (A.precision == PlasmaRealFloat)
         return launch_potrf_diag<float>(uplo, A, k, 1, 1, inv, status, s);
+    return PlasmaErrorNotSupported;
+}
+
+extern "C" int plasma_kern_potrf_diag_blocks(plasma_enum_t uplo,
+                                             plasma_desc_t A, int k,
+                                             void *inv, int *status,
+                                             hipStream_t s)
+{
+    if (A.precision == PlasmaRealDouble)
+        return launch_potrf_diag<double>(uplo, A, k, 1, 2, inv, status, s);
+    if (A.precision == PlasmaRealFloat)
+        return launch_potrf_diag<float>(uplo, A, k, 1, 2, inv, status, s);
+    return PlasmaErrorNotSupported;
+}
+
+extern "C" int plasma_panel_blocked_ok(plasma_desc_t A)
+{
+    if (A.mb != A.nb) return 0;
+    if (A.precision == PlasmaRealDouble)
+        return pb_lds_bytes<double>(A.nb) <= 160 * 1024;
+    if (A.precision == PlasmaRealFloat)
+        return pb_lds_bytes<float>(A.nb) <= 160 * 1024;
+    return 0;
+}
+
+extern "C" int plasma_kern_trsm_panel_blocked(plasma_enum_t uplo,
+                                              plasma_desc_t A, int k,
+                                              const void *inv, void *panel,
+                                              int pbase, int pcols, int pcol,
+                                              const int *status,
+                                              hipStream_t s)
+{
+    if (A.precision == PlasmaRealDouble)
+        return launch_panel_blocked<double>(uplo, A, k, inv, panel, pbase,
+                                            pcols, pcol, status, s);
+    if (A.precision == PlasmaRealFloat)
+        return launch_panel_blocked<float>(uplo, A, k, inv, panel, pbase,
+                                           pcols, pcol, status, s);
     return PlasmaErrorNotSupported;
 }
 

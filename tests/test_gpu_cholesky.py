@@ -139,6 +139,30 @@ def test_spotrf_split_update(plasma, nb_guard, uplo, n, nb, depth):
 
 
 @pytest.mark.parametrize("uplo", [L, Up])
+@pytest.mark.parametrize("lookahead", [0, 1])
+@pytest.mark.parametrize("n,nb", [(1000, 128), (1300, 256)])
+def test_spotrf_panel_schedules(plasma, nb_guard, uplo, lookahead, n, nb):
+    """FP32 without lookahead solves its panels by 32-column block
+    substitution (diagonal-block inverses only); with lookahead by the
+    full-inverse GEMM: both against the oracle, and sposv's residual (the
+    sweeps need the whole W_kk, formed after the factorization)."""
+    pa.set_param(pa.PlasmaNb, nb)
+    pa.set_param(pa.PlasmaLookahead, lookahead)
+    A = U.ggt_spd(n, seed=n + lookahead, dtype=np.float32)
+    ref, _ = ora.potrf(ora.LOWER if uplo == L else ora.UPPER, A, nb,
+                       dtype=np.float32)
+    a = A.copy(order="F")
+    assert pa.potrf(uplo, n, a, n, prec="s") == 0
+    assert U.factor_error(a, ref, uplo == L) <= 1e-5
+    B = U.rhs(n, 5, seed=4, dtype=np.float32)
+    a2, b = A.copy(order="F"), B.copy(order="F")
+    assert pa.posv(uplo, n, 5, a2, n, b, n, prec="s") == 0
+    res = U.scaled_residual(A.astype(np.float64), b.astype(np.float64),
+                            B.astype(np.float64))
+    assert res < 10 * U.EPS32
+
+
+@pytest.mark.parametrize("uplo", [L, Up])
 @pytest.mark.parametrize("prec", ["d", "s"])
 def test_potrs_wide_rhs(plasma, nb_guard, uplo, prec):
     """Sweeps with >= 1024 right-hand sides run as 128 x 128 block GEMMs:

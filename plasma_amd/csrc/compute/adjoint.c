@@ -8,22 +8,35 @@
  * df = sum_ij Abar(i,j) dA(i,j) for every symmetric perturbation dA.
  *
  * Cholesky adjoint.  With A = L L^T and Lbar the adjoint of the factor,
+ * the closed form is
  *     Abar = 1/2 L^{-T} Q L^{-1},
  * Q the symmetric matrix whose lower triangle is tril(L^T Lbar) (the
  * symmetric form of I. Murray, "Differentiation of the Cholesky
  * decomposition", 2016, eq. 10: Phi(L^T Lbar) + Phi(L^T Lbar)^T = Q, Phi
- * taking the lower triangle with the diagonal halved).  On the device:
- *     Q  = ltl_lower(L, Lbar)           triangular MFMA product, mirrored
- *     G  = L^{-T} Q                     backward sweep, n right-hand sides
- *     H  = L^{-T} G^T  (= L^{-T} Q L^{-1}, symmetric): lower triangle only,
- *                                       a third of a full sweep
- *     Abar = H / 2, mirrored            exact symmetry
- * The sweeps are the potrs kernels with the diagonal-tile inverses (block
- * GEMMs for this many right-hand sides).
+ * taking the lower triangle with the diagonal halved).  Applied to the
+ * whole matrix it costs two triangular sweeps with n right-hand sides, four
+ * times the factorization's flops; here it serves each diagonal tile only.
+ *
+ * The device runs the blocked reverse sweep (Murray 2016, section 4,
+ * blocked level-3 form, restated for a right-looking factorization), about
+ * twice the factorization's flops.  G, held in place of Lbar, ends as
+ * 2 Abar (lower triangle); for k = mt-1 .. 0, with C = L(k+1:, k) and
+ * W = L(k,k)^{-1}:
+ *     Gbar(k+1:, k) -= sym(G(k+1:, k+1:)) C        trailing update reversed
+ *     P = Gbar(k+1:, k) W                          panel solve reversed
+ *     Gbar(k,k) -= P^T C;  G(k+1:, k) = P
+ *     G(k,k) = W^T Q_k W,  Q_k symmetric with lower triangle
+ *              tril(L(k,k)^T tril(Gbar(k,k)))      the tile's own factor
+ * sym() reads the stored lower triangle (diagonal as stored); the last
+ * line is the closed form on one tile.  Abar = G / 2, mirrored (exact
+ * symmetry).  Upper storage is transposed into lower on the way in
+ * (L = U^T, Lbar = Ubar^T).
  *
  * potrs adjoint.  X = A^{-1} B with adjoint Xbar: Bbar = A^{-1} Xbar and
  * Abar -= 1/2 (Bbar X^T + X Bbar^T).
  */
+#include <string.h>
+
 #include "internal.h"
 
 static int adj_args_potrf(plasma_enum_t uplo, int n, int lda, int ldabar)
@@ -55,6 +68,107 @@ static int issue(int err, plasma_sequence_t *sequence,
     return err;
 }
 
+/* Device scratch of the blocked sweep. */
+typedef struct {
+    void *panel;      /* (mt - 1) tiles of mb x nb: P */
+    void *part;       /* ADJ_MAXSPLIT x nb x nb: Gram partial sums */
+    void *symw;       /* split-K partials of the symmetric product */
+    size_t symw_bytes;
+    void *q, *t1;     /* nb x nb: Q_k, Q_k W */
+} adj_work_t;
+
+enum { ADJ_MAXSPLIT = 32, ADJ_SYMSPLIT = 8 };
+
+static void adj_work_free(adj_work_t *w)
+{
+    plasma_dev_free(w->panel);
+    plasma_dev_free(w->part);
+    plasma_dev_free(w->symw);
+    plasma_dev_free(w->q);
+    plasma_dev_free(w->t1);
+}
+
+static int adj_work_alloc(plasma_desc_t A, adj_work_t *w)
+{
+    const size_t es = plasma_element_size(A.precision);
+    const size_t tile = (size_t)A.mb * A.nb * es;
+    memset(w, 0, sizeof(*w));
+    w->panel = plasma_dev_alloc(imax(A.mt - 1, 1) * tile);
+    w->part = plasma_dev_alloc(ADJ_MAXSPLIT * tile);
+    w->symw_bytes = (size_t)ADJ_SYMSPLIT * A.m * A.nb * es;
+    w->symw = plasma_dev_alloc(w->symw_bytes);
+    w->q = plasma_dev_alloc(tile);
+    w->t1 = plasma_dev_alloc(tile);
+    if (!w->panel || !w->part || !w->symw || !w->q || !w->t1) {
+        adj_work_free(w);
+        return PlasmaErrorOutOfMemory;
+    }
+    return PlasmaSuccess;
+}
+
+/* Algorithmic flops of the diagonal-tile inversions and the blocked sweep
+ * (symmetric product 2 m^2 kv, panel product and Gram term 2 m kv^2 each,
+ * the tile's own adjoint kv^3 / 3 + 4 kv^3). */
+static double adjoint_sweep_flops(plasma_desc_t A)
+{
+    double flops = 0.0;
+    for (int k = 0; k < A.mt; k++) {
+        const double kv = plasma_tile_nview(A, k);
+        const double rest = A.m - (double)k * A.nb - kv;
+        flops += kv * kv * kv * (1.0 / 3.0 + 1.0 / 3.0 + 4.0) +
+                 2.0 * rest * rest * kv + 4.0 * rest * kv * kv;
+    }
+    return flops;
+}
+
+/* The blocked reverse sweep on lower storage: Lb (general, lower triangle =
+ * Lbar) becomes G = 2 Abar (lower triangle). */
+static void adjoint_sweep_lower(plasma_desc_t A, plasma_desc_t Lb,
+                                const void *inv, adj_work_t *w, int *st,
+                                hipStream_t s, plasma_sequence_t *sequence,
+                                plasma_request_t *request)
+{
+    const size_t es = plasma_element_size(A.precision);
+    const int nb = A.nb;
+    plasma_desc_t Qd;
+    if (issue(plasma_desc_general_init(A.precision, w->q, nb, nb, nb, nb, 0,
+                                       0, nb, nb, &Qd),
+              sequence, request))
+        return;
+    for (int k = A.mt - 1; k >= 0 && sequence->status == PlasmaSuccess;
+         k--) {
+        const int k0 = k * nb;
+        const int kv = plasma_tile_nview(A, k);
+        const void *W = (const char *)inv + (size_t)k * A.mb * A.mb * es;
+        if (k < A.mt - 1) {
+            const int m1 = k0 + kv, rest = A.m - m1;
+            plasma_desc_t Gt = plasma_desc_view(Lb, m1, m1, rest, rest);
+            plasma_desc_t Cv = plasma_desc_view(A, m1, k0, rest, kv);
+            plasma_desc_t Rv = plasma_desc_view(Lb, m1, k0, rest, kv);
+            if (issue(plasma_kern_symm_residual(PlasmaLower, Gt, Cv, Rv,
+                                                w->symw, w->symw_bytes, st,
+                                                s),
+                      sequence, request))
+                break;
+            if (issue(plasma_kern_adjoint_panel_step(A, Lb, k, W, w->panel,
+                                                     w->part, ADJ_MAXSPLIT,
+                                                     st, s),
+                      sequence, request))
+                break;
+        }
+        plasma_desc_t Lkk = plasma_desc_view(A, k0, k0, kv, kv);
+        plasma_desc_t Gkk = plasma_desc_view(Lb, k0, k0, kv, kv);
+        plasma_desc_t Qv = plasma_desc_view(Qd, 0, 0, kv, kv);
+        if (issue(plasma_kern_ltl_lower(PlasmaLower, Lkk, Gkk, Qv, st, s),
+                  sequence, request))
+            break;
+        if (issue(plasma_kern_adjoint_diag_sandwich(Lb, k, W, w->q, w->t1,
+                                                    st, s),
+                  sequence, request))
+            break;
+    }
+}
+
 int plasma_xpotrf_adjoint(plasma_enum_t precision, plasma_enum_t uplo,
                           int n, const void *pA, int lda, void *pAbar,
                           int ldabar)
@@ -71,15 +185,16 @@ int plasma_xpotrf_adjoint(plasma_enum_t precision, plasma_enum_t uplo,
         return PlasmaSuccess;
     if (ctx->tuning)
         plasma_tune_potrf(ctx, precision, n);
-    /* the sweeps with n right-hand sides dominate: their block GEMMs want
-     * tiles of at least 256 once n is large (n = 16384, FP64: 250 ms at nb
-     * 128, 218 ms at 256) */
+    /* the symmetric product of each step wants tall panels: tiles of at
+     * least 256 once n is large */
     const int nb = (n >= 8192) ? imax(ctx->nb, 256) : ctx->nb;
     const size_t es = plasma_element_size(precision);
 
-    plasma_desc_t A, Lb, Q;
-    int retval = plasma_desc_triangular_create(precision, uplo, nb, nb, n, n,
-                                               0, 0, n, n, &A);
+    /* L (general descriptor, lower triangle read), G (in place of Lbar),
+     * S (staging of upper storage, then the symmetric result) */
+    plasma_desc_t A, Lb, S;
+    int retval = plasma_desc_general_create(precision, nb, nb, n, n, 0, 0, n,
+                                            n, &A);
     if (retval != PlasmaSuccess)
         return retval;
     retval = plasma_desc_general_create(precision, nb, nb, n, n, 0, 0, n, n,
@@ -89,26 +204,33 @@ int plasma_xpotrf_adjoint(plasma_enum_t precision, plasma_enum_t uplo,
         return retval;
     }
     retval = plasma_desc_general_create(precision, nb, nb, n, n, 0, 0, n, n,
-                                        &Q);
+                                        &S);
     if (retval != PlasmaSuccess) {
         plasma_desc_destroy(&A);
         plasma_desc_destroy(&Lb);
         return retval;
     }
     void *inv = plasma_dev_alloc(plasma_inv_bytes(A));
+    adj_work_t work;
     lapack_arg_t a, ab;
-    retval = inv ? plasma_arg_stage_in(&a, (void *)pA, lda, n, n, es, 1)
-                 : PlasmaErrorOutOfMemory;
+    retval = inv ? adj_work_alloc(A, &work) : PlasmaErrorOutOfMemory;
+    if (retval == PlasmaSuccess) {
+        retval = plasma_arg_stage_in(&a, (void *)pA, lda, n, n, es, 1);
+        if (retval != PlasmaSuccess)
+            adj_work_free(&work);
+    }
     if (retval == PlasmaSuccess) {
         retval = plasma_arg_stage_in(&ab, pAbar, ldabar, n, n, es, 1);
-        if (retval != PlasmaSuccess)
+        if (retval != PlasmaSuccess) {
             plasma_arg_stage_out(&a, 0);
+            adj_work_free(&work);
+        }
     }
     if (retval != PlasmaSuccess) {
         plasma_dev_free(inv);
         plasma_desc_destroy(&A);
         plasma_desc_destroy(&Lb);
-        plasma_desc_destroy(&Q);
+        plasma_desc_destroy(&S);
         return retval;
     }
 
@@ -117,54 +239,59 @@ int plasma_xpotrf_adjoint(plasma_enum_t precision, plasma_enum_t uplo,
     plasma_sequence_init(&sequence);
     plasma_request_init(&request);
     hipStream_t s = ctx->stream;
-    /* the solve with L^T: backward sweep */
-    const plasma_enum_t back = (uplo == PlasmaLower) ? PlasmaConjTrans
-                                                     : PlasmaNoTrans;
-
-    plasma_pxlapack2desc(uplo, a.dev, a.ld, A, &sequence, &request);
-    plasma_pxlapack2desc(PlasmaGeneral, ab.dev, ab.ld, Lb, &sequence,
-                         &request);
     int *st = plasma_sequence_dstatus(&sequence, &request);
-    if (sequence.status == PlasmaSuccess && st != NULL) {
-        int slot;
-        const double dn = n;
-        plasma_prof_begin(s, PlasmaProfAdjoint, dn * dn * dn / 3.0, 0,
-                          &slot);
-        int e = issue(plasma_kern_trtri_diag(uplo, A, inv, st, s),
-                      &sequence, &request);
-        if (!e)
-            e = issue(plasma_kern_ltl_lower(uplo, A, Lb, Q, st, s),
-                      &sequence, &request);
-        plasma_prof_end(s, slot);
-        if (!e)
-            plasma_pxtrsm_left(uplo, back, A, Q, inv, &sequence, &request);
-        if (!e && sequence.status == PlasmaSuccess) {
-            plasma_prof_begin(s, PlasmaProfAdjoint, 0,
-                              2.0 * dn * dn * es, &slot);
-            e = issue(plasma_kern_tile_axpby_t(0.0, Q, 1.0, Q, Lb, 0, st,
-                                               s),
-                      &sequence, &request);
-            plasma_prof_end(s, slot);
-        }
-        if (!e)
-            plasma_pxtrsm_left_lower(uplo, A, Lb, inv, &sequence, &request);
-        if (!e && sequence.status == PlasmaSuccess) {
-            plasma_prof_begin(s, PlasmaProfAdjoint, 0,
-                              2.0 * dn * dn * es, &slot);
-            issue(plasma_kern_tile_axpby_t(0.5, Lb, 0.5, Lb, Q, 1, st, s),
-                  &sequence, &request);
-            plasma_prof_end(s, slot);
-        }
+    const double dn = n;
+    int slot;
+    if (uplo == PlasmaLower) {
+        plasma_pxlapack2desc(PlasmaGeneral, a.dev, a.ld, A, &sequence,
+                             &request);
+        plasma_pxlapack2desc(PlasmaGeneral, ab.dev, ab.ld, Lb, &sequence,
+                             &request);
     }
-    plasma_pxdesc2lapack(PlasmaGeneral, Q, ab.dev, ab.ld, &sequence,
+    else if (sequence.status == PlasmaSuccess && st != NULL) {
+        /* L = U^T and Lbar = Ubar^T: the sweep runs on lower storage */
+        plasma_pxlapack2desc(PlasmaGeneral, a.dev, a.ld, S, &sequence,
+                             &request);
+        plasma_prof_begin(s, PlasmaProfAdjoint, 0, 2.0 * dn * dn * es,
+                          &slot);
+        issue(plasma_kern_tile_axpby_t(0.0, S, 1.0, S, A, 0, st, s),
+              &sequence, &request);
+        plasma_prof_end(s, slot);
+        plasma_pxlapack2desc(PlasmaGeneral, ab.dev, ab.ld, S, &sequence,
+                             &request);
+        plasma_prof_begin(s, PlasmaProfAdjoint, 0, 2.0 * dn * dn * es,
+                          &slot);
+        issue(plasma_kern_tile_axpby_t(0.0, S, 1.0, S, Lb, 0, st, s),
+              &sequence, &request);
+        plasma_prof_end(s, slot);
+    }
+    if (sequence.status == PlasmaSuccess && st != NULL) {
+        plasma_prof_begin(s, PlasmaProfAdjoint, adjoint_sweep_flops(A), 0,
+                          &slot);
+        if (!issue(plasma_kern_trtri_diag(PlasmaLower, A, inv, st, s),
+                   &sequence, &request))
+            adjoint_sweep_lower(A, Lb, inv, &work, st, s, &sequence,
+                                &request);
+        plasma_prof_end(s, slot);
+    }
+    if (sequence.status == PlasmaSuccess && st != NULL) {
+        /* Abar = G / 2, both triangles from G's lower one */
+        plasma_prof_begin(s, PlasmaProfAdjoint, 0, 2.0 * dn * dn * es,
+                          &slot);
+        issue(plasma_kern_tile_axpby_t(0.5, Lb, 0.5, Lb, S, 1, st, s),
+              &sequence, &request);
+        plasma_prof_end(s, slot);
+    }
+    plasma_pxdesc2lapack(PlasmaGeneral, S, ab.dev, ab.ld, &sequence,
                          &request);
     int status = plasma_sequence_sync(&sequence);
     plasma_arg_stage_out(&a, 0);
     plasma_arg_stage_out(&ab, status == PlasmaSuccess);
     plasma_dev_free(inv);
+    adj_work_free(&work);
     plasma_desc_destroy(&A);
     plasma_desc_destroy(&Lb);
-    plasma_desc_destroy(&Q);
+    plasma_desc_destroy(&S);
     return status;
 }
 

@@ -545,50 +545,6 @@ void plasma_pxtrsm_left(plasma_enum_t uplo, plasma_enum_t trans,
     plasma_dev_free(scratch);
 }
 
-/* Backward sweep L^T X = B for a square B whose solution is needed in its
- * lower triangle only (a symmetric result): the rows k of the tile columns
- * j > k lie above the diagonal, so step k solves the tile columns j <= k
- * only, a third of the full sweep's work.  The strict upper triangle of B
- * is left with partial values. */
-void plasma_pxtrsm_left_lower(plasma_enum_t uplo, plasma_desc_t A,
-                              plasma_desc_t B, const void *inv,
-                              plasma_sequence_t *sequence,
-                              plasma_request_t *request)
-{
-    if (sequence->status != PlasmaSuccess)
-        return;
-    plasma_context_t *ctx = plasma_context_self();
-    int *st = plasma_sequence_dstatus(sequence, request);
-    hipStream_t s0 = ctx->stream;
-    const int mt = B.mt;
-    size_t es = plasma_element_size(B.precision);
-    void *scratch = plasma_dev_alloc((size_t)A.mb * B.nt * B.nb * es);
-    if (scratch == NULL) {
-        plasma_request_fail(sequence, request, PlasmaErrorOutOfMemory);
-        return;
-    }
-    for (int k = mt - 1; k >= 0; k--) {
-        const plasma_desc_t Bk =
-            plasma_desc_view(B, 0, 0, B.m, imin(B.n, (k + 1) * B.nb));
-        int slot;
-        double nbk = plasma_tile_mview(B, k);
-        plasma_prof_begin(s0, PlasmaProfSweep,
-                          nbk * nbk * Bk.n + 2.0 * nbk * k * B.mb * Bk.n, 0,
-                          &slot);
-        int err = issue(plasma_kern_trsm_diag_left(uplo, 1, A, Bk, k, inv,
-                                                   scratch, st, s0),
-                        sequence, request);
-        if (!err)
-            err = issue(plasma_kern_sweep_update(uplo, 1, A, Bk, k, scratch,
-                                                 st, s0),
-                        sequence, request);
-        plasma_prof_end(s0, slot);
-        if (err)
-            break;
-    }
-    plasma_dev_free(scratch);
-}
-
 void plasma_pxpotrs(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
                     const void *inv, plasma_sequence_t *sequence,
                     plasma_request_t *request)

@@ -184,12 +184,6 @@ void plasma_pxtrsm_left(plasma_enum_t uplo, plasma_enum_t trans,
                         plasma_desc_t A, plasma_desc_t B, const void *inv,
                         plasma_sequence_t *sequence,
                         plasma_request_t *request);
-/* backward sweep of a square B whose (symmetric) solution is needed in
- * its lower triangle only */
-void plasma_pxtrsm_left_lower(plasma_enum_t uplo, plasma_desc_t A,
-                              plasma_desc_t B, const void *inv,
-                              plasma_sequence_t *sequence,
-                              plasma_request_t *request);
 /* bytes of the inverse workspace for A */
 size_t plasma_inv_bytes(plasma_desc_t A);
 /* Repeated solves with one factor: M(m,k) = L(m,k) W_kk for m > k

@@ -14,7 +14,14 @@
 //    move in contiguous runs.
 //  * syr2k_lapack_kernel: C -= s (U V^T + V U^T) on LAPACK-layout arrays
 //    (the potrs adjoint's symmetric rank-2k term, nrhs small).
+//  * Steps of the blocked reverse sweep (lower storage): PanelW (block
+//    GEMM) P(t) = Lbar(k+1+t, k) W_kk; gram_partial_kernel + gram_reduce_
+//    kernel Lbar(k,k) -= sum_t P(t)^T L(k+1+t, k), split over the tiles t
+//    into partial sums added in a fixed order; copy_panel_kernel P ->
+//    Lbar(k+1.., k); TileXW (block GEMM, twice) Lbar(k,k) = W_kk^T Q W_kk
+//    for the diagonal tile's own adjoint.
 #include "common.hip.h"
+#include "tilegemm.hip.h"
 #include "plasma_kernels.h"
 
 namespace {
@@ -217,7 +224,255 @@ __global__ __launch_bounds__(NTHREADS) void syr2k_lapack_kernel(
     C[i + (size_t)j * ldc] -= s * acc;
 }
 
+// P(t) = Lbar(k+1+t, k) W_kk: jobs (t, 128-row block, 128-column block).
+template <typename T>
+struct PanelW {
+    plasma_desc_t Lb;     // general, lower tiles hold Lbar
+    int k, bpt;
+    const T *W;           // W_kk, column major, ld mb
+    T *P;                 // panel scratch, tile t at P + t mb nb, ld mb
+    __device__ bool operator()(int b, tg::Job<T> &j) const
+    {
+        const int t = b / (bpt * bpt), sub_ = b % (bpt * bpt);
+        const int r0 = (sub_ / bpt) * tg::BM, c0 = (sub_ % bpt) * tg::BM;
+        const int mv = plasma_tile_mview(Lb, k + 1 + t);
+        const int kv = plasma_tile_nview(Lb, k);
+        if (r0 >= mv || c0 >= kv) return false;
+        const View<T> X = tile_view<T>(Lb, k + 1 + t, k);
+        j.X.p = X.p + r0;
+        j.X.rs = 1;
+        j.X.cs = X.cs;
+        j.Y.p = const_cast<T *>(W) + (size_t)c0 * Lb.mb;  // Y(c, kk) = W(kk, c)
+        j.Y.rs = Lb.mb;
+        j.Y.cs = 1;
+        j.C.p = P + (size_t)t * Lb.mb * Lb.nb + r0 + (size_t)c0 * Lb.mb;
+        j.C.rs = 1;
+        j.C.cs = Lb.mb;
+        j.xrows = mv - r0;
+        j.yrows = kv - c0;
+        j.K = kv;
+        j.lower = false;
+        j.diag_off = 0;
+        return true;
+    }
+};
+
+// part[z](i, j) = sum over tiles t of split z of P(t)^T L(k+1+t, k), 64 x 64
+// blocks of the nb x nb result (block (bi, bj) = blockIdx.x).
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void gram_partial_kernel(
+    plasma_desc_t L, int k, int ntiles, int nsplit, const T *P, T *part,
+    const int *status)
+{
+    typedef GemmNT<T, AB, AB, AKC, 32, 32> G;
+    __shared__ typename G::Smem sm;
+    if (block_should_stop(status)) return;
+    const int kv = plasma_tile_nview(L, k);
+    const int bpt = (L.nb + AB - 1) / AB;
+    const int i0 = (blockIdx.x / bpt) * AB, j0 = (blockIdx.x % bpt) * AB;
+    const int z = blockIdx.y;
+    const int t0 = (int)((long)ntiles * z / nsplit);
+    const int t1 = (int)((long)ntiles * (z + 1) / nsplit);
+    typename G::acc_t acc[G::FM][G::FN];
+    G::zero(acc);
+    if (i0 < kv && j0 < kv) {
+        for (int t = t0; t < t1; t++) {
+            const int mv = plasma_tile_mview(L, k + 1 + t);
+            View<T> X;                  // X(i, kk) = P(t)(kk, i0 + i)
+            X.p = const_cast<T *>(P) + (size_t)t * L.mb * L.nb +
+                  (size_t)i0 * L.mb;
+            X.rs = L.mb;
+            X.cs = 1;
+            const View<T> Lt = lower_view<T>(L, false, k + 1 + t, k);
+            View<T> Y;                  // Y(j, kk) = L(k+1+t, k)(kk, j0 + j)
+            Y.p = Lt.p + (ptrdiff_t)j0 * Lt.cs;
+            Y.rs = Lt.cs;
+            Y.cs = Lt.rs;
+            G::run(acc, X, kv - i0, Y, kv - j0, mv, sm);
+        }
+    }
+    T *pz = part + (size_t)z * L.nb * L.nb;     // column major, ld nb
+#pragma unroll
+    for (int i = 0; i < G::FM; i++)
+#pragma unroll
+        for (int j = 0; j < G::FN; j++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int r = i0 + G::acc_row(i, q), c = j0 + G::acc_col(j);
+                if (r < kv && c < kv) pz[(size_t)c * L.nb + r] = acc[i][j][q];
+            }
+}
+
+// Lbar(k,k) -= sum_z part[z]
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void gram_reduce_kernel(
+    plasma_desc_t Lb, int k, int nsplit, const T *part, const int *status)
+{
+    if (block_should_stop(status)) return;
+    const int kv = plasma_tile_nview(Lb, k);
+    const int e = blockIdx.x * NTHREADS + threadIdx.x;
+    if (e >= kv * kv) return;
+    const int r = e % kv, c = e / kv;
+    T sum = T(0);
+    for (int z = 0; z < nsplit; z++)
+        sum += part[(size_t)z * Lb.nb * Lb.nb + (size_t)c * Lb.nb + r];
+    tile_view<T>(Lb, k, k).at(r, c) -= sum;
+}
+
+// Lbar(k+1+t, k) = P(t)
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void copy_panel_kernel(
+    plasma_desc_t Lb, int k, const T *P, const int *status)
+{
+    if (block_should_stop(status)) return;
+    const int t = blockIdx.y;
+    const int mv = plasma_tile_mview(Lb, k + 1 + t);
+    const int kv = plasma_tile_nview(Lb, k);
+    const View<T> D = tile_view<T>(Lb, k + 1 + t, k);
+    const T *S = P + (size_t)t * Lb.mb * Lb.nb;
+    for (int e = blockIdx.x * NTHREADS + threadIdx.x; e < mv * kv;
+         e += gridDim.x * NTHREADS) {
+        const int r = e % mv, c = e / mv;
+        D.at(r, c) = S[(size_t)c * Lb.mb + r];
+    }
+}
+
+// C = op(X) W on one kv x kv tile (op(X) = X, or X^T with XT), W the
+// inverse of a diagonal tile (column major, ld ldw).
+template <typename T>
+struct TileXW {
+    View<T> X, C;
+    const T *W;
+    int ldw, kv, bpt;
+    __device__ bool operator()(int b, tg::Job<T> &j) const
+    {
+        const int r0 = (b / bpt) * tg::BM, c0 = (b % bpt) * tg::BM;
+        if (r0 >= kv || c0 >= kv) return false;
+        j.X.p = X.p + (ptrdiff_t)r0 * X.rs;
+        j.X.rs = X.rs;
+        j.X.cs = X.cs;
+        j.Y.p = const_cast<T *>(W) + (size_t)c0 * ldw;
+        j.Y.rs = ldw;
+        j.Y.cs = 1;
+        j.C.p = C.p + (ptrdiff_t)r0 * C.rs + (ptrdiff_t)c0 * C.cs;
+        j.C.rs = C.rs;
+        j.C.cs = C.cs;
+        j.xrows = kv - r0;
+        j.yrows = kv - c0;
+        j.K = kv;
+        j.lower = false;
+        j.diag_off = 0;
+        return true;
+    }
+};
+
+// Lb(k,k) = W^T Q W = (Q W)^T W (Q symmetric): T1 = Q W, then T1^T W.
+template <typename T>
+int launch_adjoint_diag_sandwich(plasma_desc_t Lb, int k, const T *W,
+                                 const T *Q, T *T1, const int *status,
+                                 hipStream_t s)
+{
+    const int kv = plasma_tile_nview(Lb, k);
+    const int bpt = (kv + tg::BM - 1) / tg::BM;
+    const int njobs = bpt * bpt;
+    const int ld = Lb.mb;
+    TileXW<T> d1;
+    d1.X.p = const_cast<T *>(Q);
+    d1.X.rs = 1;
+    d1.X.cs = ld;
+    d1.C.p = T1;
+    d1.C.rs = 1;
+    d1.C.cs = ld;
+    d1.W = W;
+    d1.ldw = Lb.mb;
+    d1.kv = kv;
+    d1.bpt = bpt;
+    hipLaunchKernelGGL((tg::kernel<T, true, false, false, true, false,
+                                   TileXW<T>>),
+                       dim3(tg::xcd_grid(njobs)), dim3(tg::NT), 0, s, d1,
+                       njobs, status);
+    TileXW<T> d2 = d1;
+    d2.X.p = T1;                    // X(r, kk) = T1(kk, r)
+    d2.X.rs = ld;
+    d2.X.cs = 1;
+    d2.C.p = (T *)plasma_tile_addr(Lb, k, k);
+    d2.C.rs = 1;
+    d2.C.cs = plasma_tile_mmain(Lb, k);
+    hipLaunchKernelGGL((tg::kernel<T, true, false, false, false, false,
+                                   TileXW<T>>),
+                       dim3(tg::xcd_grid(njobs)), dim3(tg::NT), 0, s, d2,
+                       njobs, status);
+    return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
+}
+
+template <typename T>
+int launch_adjoint_panel_step(plasma_desc_t L, plasma_desc_t Lb, int k,
+                              const T *W, T *P, T *part, int maxsplit,
+                              const int *status, hipStream_t s)
+{
+    const int ntiles = L.mt - k - 1;
+    if (ntiles <= 0) return 0;
+    // P = Lbar(k+1.., k) W_kk
+    const int bpt = (L.mb + tg::BM - 1) / tg::BM;
+    PanelW<T> d{Lb, k, bpt, W, P};
+    const int njobs = ntiles * bpt * bpt;
+    hipLaunchKernelGGL((tg::kernel<T, true, false, false, true, false,
+                                   PanelW<T>>),
+                       dim3(tg::xcd_grid(njobs)), dim3(tg::NT), 0, s, d,
+                       njobs, status);
+    // Lbar(k,k) -= sum_t P(t)^T L(k+1+t, k)
+    const int gb = (L.nb + AB - 1) / AB;
+    const int nsplit = max(1, min(ntiles, maxsplit));
+    hipLaunchKernelGGL(gram_partial_kernel<T>, dim3(gb * gb, nsplit),
+                       dim3(NTHREADS), 0, s, L, k, ntiles, nsplit,
+                       (const T *)P, part, status);
+    const int kv = plasma_tile_nview(L, k);
+    hipLaunchKernelGGL(gram_reduce_kernel<T>,
+                       dim3((kv * kv + NTHREADS - 1) / NTHREADS),
+                       dim3(NTHREADS), 0, s, Lb, k, nsplit,
+                       (const T *)part, status);
+    // Lbar(k+1.., k) = P
+    hipLaunchKernelGGL(copy_panel_kernel<T>, dim3(8, ntiles), dim3(NTHREADS),
+                       0, s, Lb, k, (const T *)P, status);
+    return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
+}
+
 }  // namespace
+
+extern "C" int plasma_kern_adjoint_panel_step(plasma_desc_t L,
+                                              plasma_desc_t Lb, int k,
+                                              const void *W, void *P,
+                                              void *part, int maxsplit,
+                                              const int *status,
+                                              hipStream_t s)
+{
+    if (L.precision != Lb.precision) return PlasmaErrorIllegalValue;
+    if (L.precision == PlasmaRealDouble)
+        return launch_adjoint_panel_step<double>(
+            L, Lb, k, (const double *)W, (double *)P, (double *)part,
+            maxsplit, status, s);
+    if (L.precision == PlasmaRealFloat)
+        return launch_adjoint_panel_step<float>(
+            L, Lb, k, (const float *)W, (float *)P, (float *)part, maxsplit,
+            status, s);
+    return PlasmaErrorNotSupported;
+}
+
+extern "C" int plasma_kern_adjoint_diag_sandwich(plasma_desc_t Lb, int k,
+                                                 const void *W, const void *Q,
+                                                 void *T1, const int *status,
+                                                 hipStream_t s)
+{
+    if (Lb.precision == PlasmaRealDouble)
+        return launch_adjoint_diag_sandwich<double>(
+            Lb, k, (const double *)W, (const double *)Q, (double *)T1,
+            status, s);
+    if (Lb.precision == PlasmaRealFloat)
+        return launch_adjoint_diag_sandwich<float>(
+            Lb, k, (const float *)W, (const float *)Q, (float *)T1, status,
+            s);
+    return PlasmaErrorNotSupported;
+}
 
 extern "C" int plasma_kern_ltl_lower(plasma_enum_t uplo, plasma_desc_t L,
                                      plasma_desc_t Lbar, plasma_desc_t Q,

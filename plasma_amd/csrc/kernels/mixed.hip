@@ -351,7 +351,7 @@ int launch_symm(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t X,
     const size_t part_bytes = (size_t)R.m * R.n * sizeof(T);
     int ksplit = (int)((SY_TARGET + blocks - 1) / blocks);
     ksplit = min(ksplit, SY_MAXSPLIT);
-    if (work == nullptr || R.i != 0 || R.j != 0) ksplit = 1;
+    if (work == nullptr || R.i % R.mb != 0 || R.j % R.nb != 0) ksplit = 1;
     else ksplit = (int)min((size_t)ksplit, work_bytes / part_bytes);
     if (ksplit < 1) ksplit = 1;
     hipLaunchKernelGGL(symm_residual_kernel<T>,

@@ -166,6 +166,22 @@ int plasma_kern_ltl_lower(plasma_enum_t uplo, plasma_desc_t L,
 int plasma_kern_tile_axpby_t(double a, plasma_desc_t X, double b,
                              plasma_desc_t Y, plasma_desc_t Z, int mirror,
                              const int *status, hipStream_t s);
+/* One panel step of the blocked reverse sweep of the Cholesky adjoint
+ * (lower storage; L the factor, Lb a general descriptor whose lower tiles
+ * hold Lbar): P = Lbar(k+1.., k) W_kk (W = the inverse of tile (k,k),
+ * column major, ld mb; P: (mt-k-1) tiles of mb x nb), Lbar(k,k) -= sum_t
+ * P(t)^T L(k+1+t, k) (split over up to maxsplit tile ranges, part: maxsplit
+ * x nb x nb elements), then Lbar(k+1.., k) = P. */
+int plasma_kern_adjoint_panel_step(plasma_desc_t L, plasma_desc_t Lb, int k,
+                                   const void *W, void *P, void *part,
+                                   int maxsplit, const int *status,
+                                   hipStream_t s);
+/* Lb(k,k) = W^T Q W for the symmetric Q (column major, ld Lb.mb, order
+ * of tile k) and W the inverse of the factor's tile (k,k); T1: mb x mb
+ * scratch. */
+int plasma_kern_adjoint_diag_sandwich(plasma_desc_t Lb, int k, const void *W,
+                                      const void *Q, void *T1,
+                                      const int *status, hipStream_t s);
 int plasma_kern_syr2k_lapack(plasma_enum_t precision, int n, int k,
                              double scale, const void *U, int ldu,
                              const void *V, int ldv, void *C, int ldc,

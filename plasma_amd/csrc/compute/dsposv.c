@@ -143,7 +143,7 @@ void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
     }
     void *inv_d = NULL, *mpan = NULL;
     /* split-K partials of the residual (optional: NULL runs unsplit) */
-    const size_t symm_bytes = 4 * (size_t)n * nrhs * sizeof(double);
+    const size_t symm_bytes = 8 * (size_t)n * nrhs * sizeof(double);
     void *symm_work = plasma_dev_alloc(symm_bytes);
     double *xnorm = work + lw;
     double *rnorm = xnorm + nrhs;

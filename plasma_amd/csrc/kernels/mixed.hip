@@ -163,6 +163,7 @@ __global__ __launch_bounds__(NTHREADS) void amax_cols_kernel(
 // of R.
 // ---------------------------------------------------------------------------
 constexpr int SY_BM = 64, SY_BN = 64, SY_KC = 16;
+constexpr int SY_WM = 32, SY_WN = 32;   // per wave: right-hand sides x rows
 
 template <typename T>
 __global__ __launch_bounds__(NTHREADS) void symm_residual_kernel(
@@ -170,7 +171,7 @@ __global__ __launch_bounds__(NTHREADS) void symm_residual_kernel(
     int rchunks, int cchunks, int ksplit, T *part, const int *status)
 {
     // first operand: 64 right-hand sides; second: 64 rows of Asym
-    typedef GemmNT<T, SY_BN, SY_BM, SY_KC, 32, 32> G;
+    typedef GemmNT<T, SY_BN, SY_BM, SY_KC, SY_WM, SY_WN> G;
     __shared__ typename G::Smem sm;
     if (block_should_stop(status)) return;
     const int m = blockIdx.x / rchunks;
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(NTHREADS) void symm_residual_kernel(
     const int mv = plasma_tile_mview(R, m), nv = plasma_tile_nview(R, tn);
     if (r0 >= mv || c0 >= nv) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wr = wave / 2, wc = wave % 2;
+    const int wr = wave / (SY_BM / SY_WN), wc = wave % (SY_BM / SY_WN);
     const int t = threadIdx.x;
     const int cpt = (A.mb + SY_KC - 1) / SY_KC;   // chunks per tile column
     const int nchunks = A.mt * cpt;
@@ -258,10 +259,10 @@ __global__ __launch_bounds__(NTHREADS) void symm_residual_kernel(
             T a[G::FM], b[G::FN];
 #pragma unroll
             for (int i = 0; i < G::FM; i++)
-                a[i] = sm.x[cur][kk + lane / 16][wr * 32 + i * 16 + lane % 16];
+                a[i] = sm.x[cur][kk + lane / 16][wr * SY_WM + i * 16 + lane % 16];
 #pragma unroll
             for (int j = 0; j < G::FN; j++)
-                b[j] = sm.y[cur][kk + lane / 16][wc * 32 + j * 16 + lane % 16];
+                b[j] = sm.y[cur][kk + lane / 16][wc * SY_WN + j * 16 + lane % 16];
 #pragma unroll
             for (int i = 0; i < G::FM; i++)
 #pragma unroll
@@ -335,10 +336,11 @@ __global__ __launch_bounds__(NTHREADS) void symm_reduce_kernel(
     C.at(row % R.mb, col % R.nb) -= sum;
 }
 
-// The grid has one workgroup per 64 x 64 block of R; when that is small
-// (few right-hand sides) and work has room, K is split so that about
-// SY_TARGET workgroups share the chip.
-constexpr int SY_TARGET = 2048, SY_MAXSPLIT = 8;
+// The grid has one workgroup per 64 x 64 block of R (column blocks only
+// over the right-hand sides present: min(nb, n) columns per tile column);
+// when that is small (few right-hand sides) and work has room, K is split
+// so that about SY_TARGET workgroups share the chip.
+constexpr int SY_TARGET = 4096, SY_MAXSPLIT = 8;
 
 template <typename T>
 int launch_symm(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t X,
@@ -346,7 +348,7 @@ int launch_symm(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t X,
                 const int *status, hipStream_t s)
 {
     const int rchunks = (R.mb + SY_BM - 1) / SY_BM;
-    const int cchunks = (R.nb + SY_BN - 1) / SY_BN;
+    const int cchunks = (min(R.nb, R.n) + SY_BN - 1) / SY_BN;
     const long blocks = (long)R.mt * rchunks * R.nt * cchunks;
     const size_t part_bytes = (size_t)R.m * R.n * sizeof(T);
     int ksplit = (int)((SY_TARGET + blocks - 1) / blocks);

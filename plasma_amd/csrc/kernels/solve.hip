@@ -270,7 +270,7 @@ int launch_sweep_diag(int backward, plasma_desc_t A, plasma_desc_t B, int k,
                         : launch_wide<T, false, true>(d, njobs, status, s);
     }
     const int rchunks = (B.mb + SW_BM - 1) / SW_BM;
-    const int cchunks = (B.nb + SW_BN - 1) / SW_BN;
+    const int cchunks = (min(B.nb, B.n) + SW_BN - 1) / SW_BN;
     hipLaunchKernelGGL(sweep_diag_kernel<T>, dim3(rchunks, B.nt * cchunks),
                        dim3(NTHREADS), 0, s, A, B, backward, k, cchunks,
                        (const T *)inv, (T *)scratch, (size_t)0, 0, 0,
@@ -285,7 +285,7 @@ int launch_sweep_update(plasma_enum_t uplo, int backward, plasma_desc_t A,
 {
     int ntiles = backward ? k : B.mt - k - 1;
     const int rchunks = (B.mb + SW_BM - 1) / SW_BM;
-    const int cchunks = (B.nb + SW_BN - 1) / SW_BN;
+    const int cchunks = (min(B.nb, B.n) + SW_BN - 1) / SW_BN;
     if (B.n >= WIDE_RHS) {
         // the update as block GEMMs; sweep_update_kernel below then only
         // copies S into B(k, :)
@@ -434,7 +434,7 @@ int launch_msweep_update(int backward, plasma_desc_t A, plasma_desc_t B,
     const int ntiles = backward ? k : B.mt - k - 1;
     if (ntiles <= 0) return 0;
     const int rchunks = (B.mb + SW_BM - 1) / SW_BM;
-    const int cchunks = (B.nb + SW_BN - 1) / SW_BN;
+    const int cchunks = (min(B.nb, B.n) + SW_BN - 1) / SW_BN;
     hipLaunchKernelGGL(msweep_update_kernel<T>,
                        dim3(ntiles * rchunks, B.nt * cchunks), dim3(NTHREADS),
                        0, s, A, B, backward, k, rchunks, cchunks,
@@ -449,7 +449,7 @@ int launch_diag_both(plasma_desc_t A, plasma_desc_t B, const void *inv,
                      void *scratch, const int *status, hipStream_t s)
 {
     const int rchunks = (B.mb + SW_BM - 1) / SW_BM;
-    const int cchunks = (B.nb + SW_BN - 1) / SW_BN;
+    const int cchunks = (min(B.nb, B.n) + SW_BN - 1) / SW_BN;
     const size_t kstride = (size_t)A.mb * B.nt * B.nb;
     dim3 grid(rchunks, B.nt * cchunks, B.mt);
     hipLaunchKernelGGL(sweep_diag_kernel<T>, grid, dim3(NTHREADS), 0, s, A, B,

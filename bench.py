@@ -19,9 +19,11 @@ scaled residual ||B - A X||_inf / (n ||A||_inf ||X||_inf eps64) < 10.
 
 Timed steps run with PlasmaGraphCapture enabled (the factorization's
 launch sequence replays from a captured hipGraph) and no per-launch
-profiling; a second, profiled pass of the same solves, launched one by one
-with HIP events around every launch, gives the per-kernel-class times and
-the roofline.  Both times are printed.
+profiling; a second, profiled pass of the same solves with the same
+schedules (capture stays enabled, so the library reads the same rows of its
+tuned table; profiling bypasses the graphs), launched one by one with HIP
+events around every launch, gives the per-kernel-class times and the
+roofline.  Both times are printed.
 
 One mixed step is one plasma_dsposv call on device-resident LAPACK arrays
 (translation to tiles, FP32 factor, refinement, translation back, final
@@ -246,17 +248,21 @@ class Bench:
 
     def schedule(self, prec, n):
         """(nb, update depth, lookahead) of a factorization of order n: the
-        pinned one, or the autotuned table's row the library applies (the
-        first row of the precision with n <= n_max, else the last)."""
+        pinned one, or the autotuned table's row the library applies in
+        graph mode, the mode of every solve here (among the rows of the
+        precision for graph replay or for any mode, else the eager rows: the
+        first with n <= n_max, else the last)."""
         if self.args.nb > 0:
             return self.args.nb, 2, self.args.lookahead
         path = os.environ.get("PLASMA_TUNING_FILE") or os.path.join(
             ROOT, "plasma_amd", "tuning", "mi355x.txt")
-        rows = []
+        rows, eager = [], []
         for line in open(path):
             f = line.split()
-            if len(f) == 6 and f[0] == "potrf" and f[1] == prec:
-                rows.append(tuple(int(x) for x in f[2:]))
+            if len(f) in (6, 7) and f[0] == "potrf" and f[1] == prec:
+                r = tuple(int(x) for x in f[2:6])
+                (eager if f[6:] == ["eager"] else rows).append(r)
+        rows = sorted(rows or eager)
         if not rows:
             raise SystemExit("no %s rows in %s" % (prec, path))
         for n_max, nb, depth, la in rows:
@@ -583,9 +589,9 @@ def main():
         ms_m, _, res_m, iters = bench.mixed(A0, B0, "mixed " + tag,
                                             profile=False)
         ms_p, _, res_p = bench.plain(A0, B0, "plain " + tag, profile=False)
-        bench.pa.set_param(bench.pa.PlasmaGraphCapture,
-                           bench.pa.PlasmaDisabled)
-        # profiled: the same solves launched one by one, every launch
+        # profiled: the same solves and schedules (graph capture stays
+        # enabled, so the library picks the same rows of its table) launched
+        # one by one, since profiling bypasses the graphs; every launch
         # bracketed by HIP events (kernel classes and the roofline)
         pms_m, prof_m, _, _ = bench.mixed(A0, B0, "mixed profiled " + tag)
         pms_p, prof_p, _ = bench.plain(A0, B0, "plain profiled " + tag)
@@ -615,8 +621,6 @@ def main():
                            bench.pa.PlasmaEnabled)
         ms_g, _, res_g = bench.plain(A0, B0, "plain G G^T/n + I",
                                      profile=False)
-        bench.pa.set_param(bench.pa.PlasmaGraphCapture,
-                           bench.pa.PlasmaDisabled)
         _, prof_g, _ = bench.plain(A0, B0, "plain profiled G G^T/n + I")
         del A0
         ggt = {"workload": "plain FP64 dposv (drop-in) n=%d nrhs=%d lower, "

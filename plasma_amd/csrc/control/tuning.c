@@ -8,11 +8,15 @@
  * absent.  Here the table is plain text, written by the tuner
  * (plasma_amd/tune.py) from measurements on the GPU:
  *
- *     # routine precision n_max nb update_depth lookahead
- *     potrf d 8192 256 2 1
+ *     # routine precision n_max nb update_depth lookahead [mode]
+ *     potrf d 8192 256 2 1 graph
  *
- * A factorization of order n in precision p uses the first row of p with
- * n <= n_max (rows sorted by n_max), or the last row of p.  Like the
+ * The optional mode, "eager" or "graph", says which launch mode the row was
+ * measured in (PlasmaGraphCapture disabled or enabled); a row without one
+ * serves both.  A factorization of order n in precision p uses, among the
+ * rows of p for the context's mode (those of the other mode when p has
+ * none), the first with n <= n_max (rows sorted by n_max), or the last.
+ * Like the
  * reference, a tuned call leaves its choice in the context (nb, update
  * depth, lookahead).  The table is read at plasma_init from
  * $PLASMA_TUNING_FILE, else from tuning/mi355x.txt next to the library's
@@ -49,15 +53,21 @@ int plasma_tuning_load(const char *path)
     char line[256];
     while (fgets(line, sizeof(line), f) != NULL &&
            ctx->ntune < PLASMA_TUNE_ROWS) {
-        char routine[32], prec;
+        char routine[32], prec, mode[16] = "";
         int n_max, nb, depth, la;
-        if (line[0] == '#' ||
-            sscanf(line, "%31s %c %d %d %d %d", routine, &prec, &n_max, &nb,
-                   &depth, &la) != 6 ||
-            strcmp(routine, "potrf") != 0)
+        const int got = sscanf(line, "%31s %c %d %d %d %d %15s", routine,
+                               &prec, &n_max, &nb, &depth, &la, mode);
+        if (line[0] == '#' || got < 6 || strcmp(routine, "potrf") != 0)
             continue;
         if ((prec != 'd' && prec != 's') || n_max < 1 || nb < 1 ||
             depth < 1 || depth > 4 || la < 0 || la > 4)
+            continue;
+        int m = 0;
+        if (got == 7 && strcmp(mode, "eager") == 0)
+            m = 1;
+        else if (got == 7 && strcmp(mode, "graph") == 0)
+            m = 2;
+        else if (got == 7)
             continue;
         plasma_tune_row_t *r = &ctx->tune[ctx->ntune++];
         r->precision = prec == 'd' ? PlasmaRealDouble : PlasmaRealFloat;
@@ -65,6 +75,7 @@ int plasma_tuning_load(const char *path)
         r->nb = nb;
         r->depth = depth;
         r->lookahead = la;
+        r->mode = m;
     }
     fclose(f);
     qsort(ctx->tune, ctx->ntune, sizeof(ctx->tune[0]), row_cmp);
@@ -95,15 +106,18 @@ void plasma_tuning_init(plasma_context_t *ctx)
 
 void plasma_tune_potrf(plasma_context_t *ctx, plasma_enum_t precision, int n)
 {
+    const int mode = ctx->graph_capture == PlasmaEnabled ? 2 : 1;
     const plasma_tune_row_t *pick = NULL;
-    for (int i = 0; i < ctx->ntune; i++) {
-        const plasma_tune_row_t *r = &ctx->tune[i];
-        if (r->precision != precision)
-            continue;
-        pick = r;
-        if (n <= r->n_max)
-            break;
-    }
+    for (int any = 0; any < 2 && pick == NULL; any++)
+        for (int i = 0; i < ctx->ntune; i++) {
+            const plasma_tune_row_t *r = &ctx->tune[i];
+            if (r->precision != precision ||
+                (!any && r->mode != 0 && r->mode != mode))
+                continue;
+            pick = r;
+            if (n <= r->n_max)
+                break;
+        }
     if (pick == NULL)
         return;
     ctx->nb = pick->nb;

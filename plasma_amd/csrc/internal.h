@@ -39,6 +39,7 @@ typedef struct {
 #define PLASMA_TUNE_ROWS 64
 typedef struct {
     int precision, n_max, nb, depth, lookahead;
+    int mode;                 /* 0 any, 1 eager launches, 2 graph replay */
 } plasma_tune_row_t;
 
 /* Cached factorization graphs (compute/pxpotrf.c). */

@@ -1,19 +1,22 @@
 """Autotuner of the factorization schedule on the GPU it runs on.
 
     python -m plasma_amd.tune [--out FILE] [--sizes 2048,4096,...]
+                              [--modes eager,graph]
 
-For each precision (d, s) and order n it times plasma_{d,s}potrf on
-device-resident A = G G^T / n + I (G iid N(0,1)) for every tile size in
-{128, 256, 512}, update depth in {1, 2, 3, 4} and lookahead in {0, 1} (the
-faster of two runs after one warm-up, tuning disabled while measuring) and
-writes
-the fastest schedule per (precision, n) as the table the library reads at
-plasma_init (control/tuning.c):
+For each launch mode (eager launches, or captured graphs replayed:
+PlasmaGraphCapture), precision (d, s) and order n it times
+plasma_{d,s}potrf on device-resident A = G G^T / n + I (G iid N(0,1)) for
+every tile size in {128, 256, 512}, update depth in {1, 2, 3, 4} and
+lookahead in {0, 1} (the faster of two runs after one warm-up, tuning
+disabled while measuring) and writes the fastest schedule per (mode,
+precision, n) as the table the library reads at plasma_init
+(control/tuning.c):
 
-    potrf <d|s> <n_max> <nb> <update_depth> <lookahead>
+    potrf <d|s> <n_max> <nb> <update_depth> <lookahead> <eager|graph>
 
 Each row covers the orders up to its n; orders above the largest measured
-n use the last row.
+n use the last row.  The two modes differ: a replayed graph has no launch
+gaps, which favours deeper lookahead.
 """
 import argparse
 import ctypes
@@ -32,6 +35,10 @@ def main():
     ap.add_argument("--nbs", default="128,256,512")
     ap.add_argument("--depths", default="1,2,3,4")
     ap.add_argument("--lookaheads", default="0,1")
+    ap.add_argument("--modes", default="eager,graph",
+                    help="launch modes to tune: eager (PlasmaGraphCapture "
+                         "disabled) and/or graph (enabled, captured graphs "
+                         "replayed)")
     args = ap.parse_args()
     import torch
     sys.path.insert(0, os.path.dirname(HERE))
@@ -44,8 +51,14 @@ def main():
     nbs = [int(x) for x in args.nbs.split(",")]
     depths = [int(x) for x in args.depths.split(",")]
     las = [int(x) for x in args.lookaheads.split(",")]
+    modes = args.modes.split(",")
     rows = []
-    for prec, dt in (("d", torch.float64), ("s", torch.float32)):
+    for mode, prec, dt in [(m, p, t) for m in modes
+                           for p, t in (("d", torch.float64),
+                                        ("s", torch.float32))]:
+        pa.set_param(pa.PlasmaGraphCapture,
+                     pa.PlasmaEnabled if mode == "graph"
+                     else pa.PlasmaDisabled)
         potrf = getattr(lib, "plasma_%spotrf" % prec)
         for n in sizes:
             g = torch.Generator(device="cuda").manual_seed(n)
@@ -73,18 +86,20 @@ def main():
                                 raise SystemExit("potrf failed: %d" % rc)
                         t = sorted(times[1:])[0]
                         flops = n ** 3 / 3.0
-                        print("tune %s n=%d nb=%d depth=%d lookahead=%d "
+                        print("tune %s %s n=%d nb=%d depth=%d lookahead=%d "
                               "%.3f ms %.2f TFLOP/s" % (
-                                  prec, n, nb, depth, la, t * 1e3,
+                                  mode, prec, n, nb, depth, la, t * 1e3,
                                   flops / t / 1e12), flush=True)
                         if best is None or t < best[0]:
                             best = (t, nb, depth, la)
-            rows.append("potrf %s %d %d %d %d" % ((prec, n) + best[1:]))
+            rows.append("potrf %s %d %d %d %d %s" % ((prec, n) + best[1:]
+                                                       + (mode,)))
             del A, A0
     lib.plasma_finalize()
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     with open(args.out, "w") as f:
-        f.write("# routine precision n_max nb update_depth lookahead\n")
+        f.write("# routine precision n_max nb update_depth lookahead "
+                "mode\n")
         f.write("# written by plasma_amd/tune.py (the faster of two timed "
                 "runs per schedule)\n")
         for r in rows:

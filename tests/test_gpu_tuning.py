@@ -57,3 +57,39 @@ def test_disabled_keeps_context(plasma, nb_guard, tmp_path):
         assert pa.get_param(pa.PlasmaNb)[1] == 128
     finally:
         lib.plasma_tuning_load(None)
+
+
+def test_table_rows_per_launch_mode(plasma, nb_guard, tmp_path):
+    """Rows tagged eager / graph apply in their launch mode
+    (PlasmaGraphCapture); untagged rows serve both; a precision with rows of
+    the other mode only falls back to them; an unknown tag drops the row."""
+    lib = pa.load()
+    table = tmp_path / "t.txt"
+    table.write_text("potrf d 5000 64 3 0 eager\n"
+                     "potrf d 5000 96 2 1 graph\n"
+                     "potrf d 9000 32 1 1 sometimes\n"
+                     "potrf s 5000 128 4 2 graph\n")
+    assert lib.plasma_tuning_load(str(table).encode()) == 0
+    try:
+        pa.set_param(pa.PlasmaTuning, pa.PlasmaEnabled)
+        ora.set_blas(ora.MKL)
+        n = 700
+        A = U.tester_spd(n, n)
+        for graph, prec, want in ((0, "d", (64, 3, 0)), (1, "d", (96, 2, 1)),
+                                  (0, "s", (128, 4, 2)),
+                                  (1, "s", (128, 4, 2))):
+            pa.set_param(pa.PlasmaGraphCapture,
+                         pa.PlasmaEnabled if graph else pa.PlasmaDisabled)
+            a = (A if prec == "d" else A.astype(np.float32)).copy(order="F")
+            assert pa.potrf(L, n, a, n, prec=prec) == 0
+            got = (pa.get_param(pa.PlasmaNb)[1],
+                   pa.get_param(pa.PlasmaUpdateDepth)[1],
+                   pa.get_param(pa.PlasmaLookahead)[1])
+            assert got == want, (graph, prec, got)
+            if prec == "d":
+                ref, _ = ora.potrf(L, A, want[0])
+                assert U.factor_error(a, ref, True) <= 1e-12
+    finally:
+        lib.plasma_tuning_load(None)
+        pa.set_param(pa.PlasmaTuning, pa.PlasmaDisabled)
+        pa.set_param(pa.PlasmaGraphCapture, pa.PlasmaDisabled)

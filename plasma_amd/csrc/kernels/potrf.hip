@@ -14,10 +14,21 @@
 namespace {
 
 constexpr int NTHREADS = 256;
-// threads of the diagonal-tile factorization: 8 waves in FP32; FP64's
-// block code needs more than the 256 registers per lane 8 waves leave
-template <typename T> struct DiagNT { static constexpr int v = 512; };
-template <> struct DiagNT<double> { static constexpr int v = 256; };
+// Threads of the diagonal-tile factorization and the workgroups per CU its
+// register budget allows.  It runs beside the trailing update, whose
+// workgroups take 256 registers per lane on each SIMD (one wave each): a
+// workgroup that needs the whole register file of its CU can only start on
+// a CU the update has left entirely, i.e. near the update's end.  FP32 runs
+// 4 waves within 256 registers (min 2 per CU), so it fits beside one update
+// workgroup (n = 32768 eager, lookahead 1: factor 134.7 -> 125.2 ms).
+// FP64's block code needs 341 registers; bounded to 256 it spills and gains
+// nothing, so it keeps them.
+template <typename T> struct DiagNT {
+    static constexpr int v = 256, min_blocks = 2;
+};
+template <> struct DiagNT<double> {
+    static constexpr int v = 256, min_blocks = 1;
+};
 constexpr int JB = 32;          // column block of the diagonal tile
 
 // ---------------------------------------------------------------------------
@@ -525,8 +536,8 @@ __device__ int factor_tile(const View<T> &L, const int n, const View<T> &Wv,
 // column j sets the status word to k*nb + j + 1 (reference
 // core_zpotrf.c:72-79, iinfo = A.nb * k).
 // ---------------------------------------------------------------------------
-template <typename T, int NT>
-__global__ __launch_bounds__(NT) void potrf_diag_kernel(
+template <typename T, int NT, int MINB = 1>
+__global__ __launch_bounds__(NT, MINB) void potrf_diag_kernel(
     plasma_desc_t A, int upper, int k0, int factor, T *inv, int *status)
 {
     if (block_should_stop(status)) return;
@@ -671,8 +682,10 @@ int launch_potrf_diag(plasma_enum_t uplo, plasma_desc_t A, int k0, int count,
     if (lds + lds_static > 160 * 1024) return PlasmaErrorNotSupported;
     // the factorization (one tile, on the critical path) runs 8 waves; the
     // invert-only form (a batch of tiles) 4 per tile
-    const void *fn = factor ? (const void *)potrf_diag_kernel<T, DiagNT<T>::v>
-                            : (const void *)potrf_diag_kernel<T, NTHREADS>;
+    const void *fn =
+        factor ? (const void *)potrf_diag_kernel<T, DiagNT<T>::v,
+                                                 DiagNT<T>::min_blocks>
+               : (const void *)potrf_diag_kernel<T, NTHREADS>;
     if (lds > 65536) {
         static int attr_set[2][2] = {{0, 0}, {0, 0}};
         int &done = attr_set[sizeof(T) == 8][factor != 0];
@@ -683,7 +696,9 @@ int launch_potrf_diag(plasma_enum_t uplo, plasma_desc_t A, int k0, int count,
         }
     }
     if (factor)
-        hipLaunchKernelGGL((potrf_diag_kernel<T, DiagNT<T>::v>), dim3(count),
+        hipLaunchKernelGGL((potrf_diag_kernel<T, DiagNT<T>::v,
+                                              DiagNT<T>::min_blocks>),
+                           dim3(count),
                            dim3(DiagNT<T>::v), lds, s, A, uplo == PlasmaUpper, k0,
                            factor, (T *)inv, status);
     else

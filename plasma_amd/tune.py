@@ -35,6 +35,8 @@ def main():
     ap.add_argument("--nbs", default="128,256,512")
     ap.add_argument("--depths", default="1,2,3,4")
     ap.add_argument("--lookaheads", default="0,1")
+    ap.add_argument("--precs", default="d,s",
+                    help="precisions to tune (d, s)")
     ap.add_argument("--modes", default="eager,graph",
                     help="launch modes to tune: eager (PlasmaGraphCapture "
                          "disabled) and/or graph (enabled, captured graphs "
@@ -53,9 +55,10 @@ def main():
     las = [int(x) for x in args.lookaheads.split(",")]
     modes = args.modes.split(",")
     rows = []
+    precs = args.precs.split(",")
     for mode, prec, dt in [(m, p, t) for m in modes
                            for p, t in (("d", torch.float64),
-                                        ("s", torch.float32))]:
+                                        ("s", torch.float32)) if p in precs]:
         pa.set_param(pa.PlasmaGraphCapture,
                      pa.PlasmaEnabled if mode == "graph"
                      else pa.PlasmaDisabled)

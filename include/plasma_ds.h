@@ -40,6 +40,21 @@ void plasma_omp_dsposv(plasma_enum_t uplo,
                        double *W, double *Rnorm, double *Xnorm, int *iter,
                        plasma_sequence_t *sequence, plasma_request_t *request);
 
+/* No reference counterpart (a mixed solver from a lower precision than
+ * zcposv's): the FP64 solution of A X = B from a bf16-input factor.  A is
+ * equilibrated symmetrically (S A S, S = diag(a_ii^{-1/2})), the scaled
+ * matrix is factored in FP32 with its trailing updates on bf16 MFMA from
+ * bf16 inputs (FP32 accumulation), and the solution is refined by GMRES-IR:
+ * FP64 GMRES on M^{-1} A with the factor as preconditioner M, restarted
+ * every 20 inner iterations, at most 10 refinement steps, with dsposv's
+ * stopping test.  A factor that breaks down or a refinement that does not
+ * converge falls back to the FP64 factorization.  *iter returns the total
+ * inner GMRES iterations, or -1 after the fallback.  Arguments, return
+ * codes and memory rules as plasma_dsposv (-10: iter is NULL). */
+int plasma_dhposv(plasma_enum_t uplo, int n, int nrhs,
+                  double *pA, int lda, double *pB, int ldb,
+                  double *pX, int ldx, int *iter);
+
 /* reference compute/zlag2c.c:59 -- returns 1 if an entry overflows FP32 */
 int plasma_dlag2s(int m, int n, double *pA, int lda, float *pAs, int ldas);
 /* reference compute/clag2z.c:58 */

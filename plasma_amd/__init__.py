@@ -111,6 +111,8 @@ for _p, _t in (("d", "double"), ("s", "float")):
 PROTOTYPES.update({
     "plasma_dsposv": (c_int, [c_int, c_int, c_int, c_void_p, c_int, c_void_p,
                               c_int, c_void_p, c_int, _PI]),
+    "plasma_dhposv": (c_int, [c_int, c_int, c_int, c_void_p, c_int, c_void_p,
+                              c_int, c_void_p, c_int, _PI]),
     "plasma_omp_dsposv": (None, [c_int, _D, _D, _D, _D, _D, _D, c_void_p,
                                  c_void_p, c_void_p, _PI, _PS, _PR]),
     "plasma_dlag2s": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_int]),
@@ -227,5 +229,13 @@ def potrs_adjoint(uplo, n, nrhs, a, lda, x, ldx, xbar, ldxbar, abar, ldabar,
 def dsposv(uplo, n, nrhs, a, lda, b, ldb, x, ldx):
     it = c_int(0)
     rc = load().plasma_dsposv(uplo, n, nrhs, _ptr(a), lda, _ptr(b), ldb,
+                              _ptr(x), ldx, ctypes.byref(it))
+    return rc, it.value
+
+
+def dhposv(uplo, n, nrhs, a, lda, b, ldb, x, ldx):
+    """plasma_dhposv: (return code, total inner GMRES iterations or -1)."""
+    it = c_int(0)
+    rc = load().plasma_dhposv(uplo, n, nrhs, _ptr(a), lda, _ptr(b), ldb,
                               _ptr(x), ldx, ctypes.byref(it))
     return rc, it.value

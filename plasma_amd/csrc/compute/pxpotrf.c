@@ -94,12 +94,17 @@ typedef struct {
 } group_t;
 
 /* FP32 factorizations whose tiles are a multiple of 32 run their trailing
- * updates on bf16 MFMA with split operands (when enabled) */
+ * updates on bf16 MFMA: 1 with split operands (six products, when
+ * enabled), 2 with bf16 inputs (one product, the low-precision factor of
+ * plasma_dhposv), 0 on FP32 MFMA */
 static int use_split(plasma_desc_t A)
 {
     plasma_context_t *ctx = plasma_context_self();
-    return ctx->split_update == PlasmaEnabled &&
-           A.precision == PlasmaRealFloat && A.nb % 32 == 0 && A.mb == A.nb;
+    if (A.precision != PlasmaRealFloat || A.nb % 32 != 0 || A.mb != A.nb)
+        return 0;
+    if (ctx->bf16_factor)
+        return 2;
+    return ctx->split_update == PlasmaEnabled;
 }
 
 /* Bytes of the FP32 part of a panel buffer, rounded up so that the split
@@ -163,7 +168,7 @@ static int k_update(const group_t *g, int pbase, int ncols, int n0, int n1,
         ? plasma_kern_trailing_update_split(g->uplo, g->A, pbase, kdepth, n0,
                                             n1, panel, planes_of(g, panel),
                                             g->pcols, copy_cols, max_wgs,
-                                            g->st, s)
+                                            g->split == 2, g->st, s)
         : plasma_kern_trailing_update(g->uplo, g->A, pbase, kdepth, n0, n1,
                                       panel, g->pcols, copy_cols, max_wgs,
                                       g->st, s);

@@ -59,6 +59,7 @@ static void context_defaults(plasma_context_t *ctx)
     ctx->lookahead = 1;
     ctx->group = 2;
     ctx->split_update = PlasmaEnabled;
+    ctx->bf16_factor = 0;
     ctx->graph_capture = PlasmaDisabled;
 }
 

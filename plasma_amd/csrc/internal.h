@@ -79,6 +79,8 @@ struct plasma_context_s {
     int group;            /* tile columns whose panels update together */
     int split_update;     /* FP32 trailing updates on bf16 MFMA: panels
                              split into three bf16 terms, six products */
+    int bf16_factor;      /* set by plasma_dhposv around its factorization:
+                             FP32 trailing updates from bf16 inputs */
     int graph_capture;    /* replay cached hipGraphs for the sync drivers */
 
     /* device */

@@ -182,6 +182,31 @@ int plasma_kern_adjoint_panel_step(plasma_desc_t L, plasma_desc_t Lb, int k,
 int plasma_kern_adjoint_diag_sandwich(plasma_desc_t Lb, int k, const void *W,
                                       const void *Q, void *T1,
                                       const int *status, hipStream_t s);
+/* plasma_dhposv (kernels/gmres.hip, FP64): symmetric diagonal
+ * equilibration s_i = a_ii^{-1/2} (1 where a_ii <= 0), A <- S A S on the
+ * stored tiles, B <- S B; and the column-wise vector operations of GMRES
+ * with nrhs right-hand sides in lockstep.  A vector set holds the storage
+ * pointers of up to PLASMA_VECSET_MAX descriptors with the geometry of the
+ * descriptor passed beside it.
+ *   coldots:    out[i * nrhs + c] = sum_r V_i(r, c) Z(r, c)
+ *   colcombine: Y = beta Y + alpha sum_i coef[i * nrhs + c] V_i (beta = 0
+ *               does not read Y; coef on the device) */
+#define PLASMA_VECSET_MAX 32
+typedef struct {
+    const double *p[PLASMA_VECSET_MAX];
+    int nv;
+} plasma_vecset_t;
+int plasma_kern_equil_diag(plasma_enum_t uplo, plasma_desc_t A, double *s,
+                           const int *status, hipStream_t st);
+int plasma_kern_scale_sym(plasma_enum_t uplo, plasma_desc_t A,
+                          const double *s, const int *status, hipStream_t st);
+int plasma_kern_scale_rows(plasma_desc_t B, const double *s,
+                           const int *status, hipStream_t st);
+int plasma_kern_coldots(plasma_desc_t Z, plasma_vecset_t V, double *out,
+                        const int *status, hipStream_t st);
+int plasma_kern_colcombine(plasma_desc_t Y, double beta, double alpha,
+                           plasma_vecset_t V, const double *coef,
+                           const int *status, hipStream_t st);
 int plasma_kern_syr2k_lapack(plasma_enum_t precision, int n, int k,
                              double scale, const void *U, int ldu,
                              const void *V, int ldv, void *C, int ldc,
@@ -192,7 +217,8 @@ int plasma_kern_syr2k_lapack(plasma_enum_t precision, int n, int k,
  * columns of step k are split into three bf16 planes per panel tile
  * (plasma_split_bytes(A, pcols) bytes per panel buffer, row stride
  * plasma_split_pstride(pcols)); the update then forms each product from six
- * bf16 products summed in FP32.  kdepth must be a multiple of 32.
+ * bf16 products summed in FP32, or with `single` from the leading terms
+ * alone (bf16 inputs, FP32 accumulation).  kdepth must be a multiple of 32.
  */
 size_t plasma_split_pstride(int pcols);
 size_t plasma_split_bytes(plasma_desc_t A, int pcols);
@@ -203,7 +229,7 @@ int plasma_kern_trailing_update_split(plasma_enum_t uplo, plasma_desc_t A,
                                       int pbase, int kdepth, int n_first,
                                       int n_end, const void *panel,
                                       const void *planes, int pcols,
-                                      int copy_cols, int max_wgs,
+                                      int copy_cols, int max_wgs, int single,
                                       const int *status, hipStream_t s);
 
 /*

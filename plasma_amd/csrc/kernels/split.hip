@@ -138,6 +138,7 @@ struct SplitUpdate {
     int ucols;                       // updated tile columns from n_first
     const float *panel;
     const __bf16 *planes;
+    int single;                      // one product (bf16 inputs): hi x hi
 };
 
 // Operand chunk of one workgroup: 3 planes x 128 rows x 32 k; thread t
@@ -148,13 +149,14 @@ struct Stage {
 
 __device__ __forceinline__ void load_stage(Stage &g, const __bf16 *base,
                                            int mb, int pstride, int t,
-                                           int nrows, int k0)
+                                           int nrows, int k0, int nq)
 {
     const int row = threadIdx.x >> 1, kh = (threadIdx.x & 1) * 16;
     const int r = min(row, nrows - 1);
     const bool ok = row < nrows;
 #pragma unroll
     for (int q = 0; q < 3; q++) {
+        if (q >= nq) break;
         const __bf16 *p = base + plane_off(t, q, mb, r, pstride) + k0 + kh;
 #pragma unroll
         for (int h = 0; h < 2; h++) {
@@ -165,13 +167,15 @@ __device__ __forceinline__ void load_stage(Stage &g, const __bf16 *base,
     }
 }
 
-__device__ __forceinline__ void store_stage(const Stage &g, __bf16 *s)
+__device__ __forceinline__ void store_stage(const Stage &g, __bf16 *s,
+                                            int nq)
 {
     const int row = threadIdx.x >> 1, kh = (threadIdx.x & 1) * 16;
 #pragma unroll
     for (int q = 0; q < 3; q++)
 #pragma unroll
         for (int h = 0; h < 2; h++)
+            if (q < nq)
             *reinterpret_cast<bf16x8 *>(s + (q * BM + row) * LDP + kh +
                                         8 * h) = g.v[q][h];
 }
@@ -218,17 +222,20 @@ __device__ __forceinline__ void run_split(const SplitUpdate &u, int tx_,
     const __bf16 *xb = u.planes + (size_t)xr0 * u.pstride;
     const __bf16 *yb = u.planes + (size_t)yr0 * u.pstride;
     const int nk = u.kdepth / KC;
+    const int nq = u.single ? 1 : 3;     // planes read
     Stage gx, gy;
-    load_stage(gx, xb, u.A.mb, u.pstride, tx_, xrows, 0);
-    load_stage(gy, yb, u.A.mb, u.pstride, ty_, yrows, 0);
+    load_stage(gx, xb, u.A.mb, u.pstride, tx_, xrows, 0, nq);
+    load_stage(gy, yb, u.A.mb, u.pstride, ty_, yrows, 0, nq);
     for (int kc = 0; kc < nk; kc++) {
         __syncthreads();                 // the previous chunk's reads done
-        store_stage(gx, sx);
-        store_stage(gy, sy);
+        store_stage(gx, sx, nq);
+        store_stage(gy, sy, nq);
         __syncthreads();
         if (kc + 1 < nk) {
-            load_stage(gx, xb, u.A.mb, u.pstride, tx_, xrows, (kc + 1) * KC);
-            load_stage(gy, yb, u.A.mb, u.pstride, ty_, yrows, (kc + 1) * KC);
+            load_stage(gx, xb, u.A.mb, u.pstride, tx_, xrows, (kc + 1) * KC,
+                       nq);
+            load_stage(gy, yb, u.A.mb, u.pstride, ty_, yrows, (kc + 1) * KC,
+                       nq);
         }
         const int ko = 8 * (lane / 16);
         bf16x8 a[4][3], b[4][3];
@@ -236,14 +243,16 @@ __device__ __forceinline__ void run_split(const SplitUpdate &u, int tx_,
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int q = 0; q < 3; q++)
-                a[i][q] = *reinterpret_cast<const bf16x8 *>(
-                    sx + (q * BM + wr * 64 + i * 16 + lane % 16) * LDP + ko);
+                a[i][q] = q < nq ? *reinterpret_cast<const bf16x8 *>(
+                    sx + (q * BM + wr * 64 + i * 16 + lane % 16) * LDP + ko)
+                                 : bf16x8{};
 #pragma unroll
         for (int j = 0; j < 4; j++)
 #pragma unroll
             for (int q = 0; q < 3; q++)
-                b[j][q] = *reinterpret_cast<const bf16x8 *>(
-                    sy + (q * BM + wc * 64 + j * 16 + lane % 16) * LDP + ko);
+                b[j][q] = q < nq ? *reinterpret_cast<const bf16x8 *>(
+                    sy + (q * BM + wc * 64 + j * 16 + lane % 16) * LDP + ko)
+                                 : bf16x8{};
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -252,12 +261,14 @@ __device__ __forceinline__ void run_split(const SplitUpdate &u, int tx_,
                 const bf16x8 *f = CROW ? b[j] : a[i];
                 const bf16x8 *s = CROW ? a[i] : b[j];
                 f32x4 v = acc[i][j];
-                v = mma(f[2], s[0], v);   // lo * hi
-                v = mma(f[0], s[2], v);   // hi * lo
-                v = mma(f[1], s[1], v);   // mid * mid
-                v = mma(f[1], s[0], v);   // mid * hi
-                v = mma(f[0], s[1], v);   // hi * mid
-                v = mma(f[0], s[0], v);   // hi * hi
+                if (!u.single) {
+                    v = mma(f[2], s[0], v);   // lo * hi
+                    v = mma(f[0], s[2], v);   // hi * lo
+                    v = mma(f[1], s[1], v);   // mid * mid
+                    v = mma(f[1], s[0], v);   // mid * hi
+                    v = mma(f[0], s[1], v);   // hi * mid
+                }
+                v = mma(f[0], s[0], v);       // hi * hi
                 acc[i][j] = v;
             }
     }
@@ -363,7 +374,7 @@ extern "C" int plasma_kern_split_panel(plasma_desc_t A, int k, int pbase,
 extern "C" int plasma_kern_trailing_update_split(
     plasma_enum_t uplo, plasma_desc_t A, int pbase, int kdepth, int n_first,
     int n_end, const void *panel, const void *planes, int pcols,
-    int copy_cols, int max_wgs, const int *status, hipStream_t s)
+    int copy_cols, int max_wgs, int single, const int *status, hipStream_t s)
 {
     if (A.precision != PlasmaRealFloat || kdepth % KC != 0)
         return PlasmaErrorNotSupported;
@@ -379,7 +390,7 @@ extern "C" int plasma_kern_trailing_update_split(
     SplitUpdate u{A, pbase, pcols, kdepth, copy_cols > 0 ? copy_cols : 1,
                   n_first, bpt, uplo == PlasmaUpper, ncopy,
                   (int)plasma_split_pstride(pcols), max(n_end - n_first, 0),
-                  (const float *)panel, (const __bf16 *)planes};
+                  (const float *)panel, (const __bf16 *)planes, single};
     const long padded = tg::xcd_grid((int)njobs);
     const long grid = max_wgs > 0 ? min(padded, (long)max_wgs) : padded;
     if (uplo == PlasmaUpper)

@@ -32,9 +32,11 @@ device-resident arrays (tr2desc + ge2desc, tile Cholesky, two sweeps,
 desc2tr + desc2ge); the input arrays are only read, so every step solves
 the same system.
 
-Also printed: the dominant kernel's roofline (the FP32 trailing update of
-the mixed solve; HIP events on the stream it runs on), per-kernel-class
-times, and the CPU baseline: the oracle's dsposv (PLASMA's tile algorithm
+Also printed: per kappa, plasma_dhposv on the same system (bf16-input
+factor of the equilibrated matrix, GMRES-IR; inner GMRES iterations, -1 for
+the FP64 fallback); the dominant kernel's roofline (the FP32 trailing
+update of the mixed solve; HIP events on the stream it runs on),
+per-kernel-class times, and the CPU baseline: the oracle's dsposv (PLASMA's tile algorithm
 restated, OpenMP tasks, sequential MKL tile kernels) timed on this host on
 a smaller n.
 """
@@ -345,20 +347,21 @@ class Bench:
         res = scaled_residual(A0, X, B0)
         return ms, prof, res
 
-    def mixed(self, A0, B0, label, profile=True):
-        """plasma_dsposv on device-resident LAPACK arrays."""
+    def mixed(self, A0, B0, label, profile=True, routine="plasma_dsposv"):
+        """plasma_dsposv (or plasma_dhposv, same arguments) on
+        device-resident LAPACK arrays."""
         torch, pa, lib = self.torch, self.pa, self.lib
+        fn = getattr(lib, routine)
         n, nrhs = A0.shape[0], B0.shape[0]
         X = torch.empty_like(B0)
         it = ctypes.c_int(0)
         iters = []
 
         def step():
-            rc = lib.plasma_dsposv(pa.PlasmaLower, n, nrhs, A0.data_ptr(), n,
-                                   B0.data_ptr(), n, X.data_ptr(), n,
-                                   ctypes.byref(it))
+            rc = fn(pa.PlasmaLower, n, nrhs, A0.data_ptr(), n,
+                    B0.data_ptr(), n, X.data_ptr(), n, ctypes.byref(it))
             if rc != 0:
-                raise SystemExit("plasma dsposv failed: %d" % rc)
+                raise SystemExit("%s failed: %d" % (routine, rc))
             iters.append(it.value)
 
         ms, prof = self.timed(step, label, profile)
@@ -589,6 +592,10 @@ def main():
         ms_m, _, res_m, iters = bench.mixed(A0, B0, "mixed " + tag,
                                             profile=False)
         ms_p, _, res_p = bench.plain(A0, B0, "plain " + tag, profile=False)
+        # the bf16-input factor with GMRES-IR on the same system
+        ms_h, _, res_h, it_h = bench.mixed(A0, B0, "bf16 gmres-ir " + tag,
+                                           profile=False,
+                                           routine="plasma_dhposv")
         # profiled: the same solves and schedules (graph capture stays
         # enabled, so the library picks the same rows of its table) launched
         # one by one, since profiling bypasses the graphs; every launch
@@ -607,6 +614,10 @@ def main():
             "speedup": round(ms_p / ms_m, 3),
             "profiled_mixed_ms": round(pms_m, 3),
             "profiled_plain_ms": round(pms_p, 3),
+            "bf16_gmres_ms": round(ms_h, 3),
+            "bf16_gmres_tflops": round(fl / (ms_h * 1e-3) / 1e12, 3),
+            "bf16_gmres_inner_iterations": it_h,
+            "bf16_gmres_residual_eps": round(res_h, 4),
         }
         head = (kappa, ms_m, prof_m, res_m, iters, ms_p, prof_p, res_p,
                 pms_m, pms_p)

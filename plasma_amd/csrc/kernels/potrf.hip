@@ -14,20 +14,18 @@
 namespace {
 
 constexpr int NTHREADS = 256;
-// Threads of the diagonal-tile factorization and the workgroups per CU its
-// register budget allows.  It runs beside the trailing update, whose
-// workgroups take 256 registers per lane on each SIMD (one wave each): a
-// workgroup that needs the whole register file of its CU can only start on
-// a CU the update has left entirely, i.e. near the update's end.  FP32 runs
-// 4 waves within 256 registers (min 2 per CU), so it fits beside one update
-// workgroup (n = 32768 eager, lookahead 1: factor 134.7 -> 125.2 ms).
-// FP64's block code needs 341 registers; bounded to 256 it spills and gains
-// nothing, so it keeps them.
+// Threads of the diagonal-tile factorization and its minimum waves per
+// SIMD (the second __launch_bounds__ argument).  It runs beside the
+// trailing update, whose workgroups take 256 registers per lane on each
+// SIMD (one wave each): a workgroup that needs the whole register file of
+// its CU can only start on a CU the update has left entirely, i.e. near the
+// update's end.  Both precisions run 4 waves held to 256 registers (at
+// least 2 waves per SIMD), so they fit beside one update workgroup.  FP32
+// fits without spills (n = 32768 eager, lookahead 1: factor 134.7 ->
+// 125.2 ms); FP64 spills 10 registers and still gains (depth 3: 244.9 ->
+// 236.2 ms eager, 240.1 -> 234.8 ms graph).
 template <typename T> struct DiagNT {
-    static constexpr int v = 256, min_blocks = 2;
-};
-template <> struct DiagNT<double> {
-    static constexpr int v = 256, min_blocks = 1;
+    static constexpr int v = 256, min_waves = 2;
 };
 constexpr int JB = 32;          // column block of the diagonal tile
 
@@ -88,7 +86,8 @@ __device__ __forceinline__ double lane_value(double x, int src)
 // readlane, every lane updates all of its trailing columns (entries above
 // the diagonal receive values that are never read: the caller keeps the
 // lower triangle only), and a failed pivot is recorded, not returned from
-// inside the loop.  sb: 2 * JB elements of LDS (column j of L broadcast).
+// inside the loop.  Column j of L reaches the other lanes by readlane; sb
+// (2 * JB elements of LDS) is not used.
 template <typename T>
 __device__ __forceinline__ int wave_potrf32(T (&d)[JB], int jb, T *sb)
 {
@@ -102,39 +101,36 @@ __device__ __forceinline__ int wave_potrf32(T (&d)[JB], int jb, T *sb)
         const T rl = pivot_rcp(ljj);
         const T lrj = (r == j) ? ljj : d[j] * rl;
         d[j] = lrj;
-        sb[JB + r] = lrj;                        // column j of L
-        wave_sync_lds();
+        // column j of L from its lanes by readlane (lane c holds L(c, j)):
+        // no LDS round trip on the pivot chain
 #pragma unroll
-        for (int c = j + 1; c < JB; c++) d[c] -= lrj * sb[JB + c];
+        for (int c = j + 1; c < JB; c++) d[c] -= lrj * lane_value(lrj, c);
     }
-    wave_sync_lds();
+    (void)sb;
     return info;
 }
 
 // Inverse of the lower-triangular block held in d (rows on lanes): lane l
-// returns column l % 32 of L^{-1} in inv[0..31].  sl: JB x LDB elements of
-// LDS that receive the block row by row.
+// returns column l % 32 of L^{-1} in inv[0..31].  Row r of L comes from
+// lane r by readlane, so nothing goes through LDS (sl is not used).
 template <typename T>
 __device__ __forceinline__ void wave_trtri32(const T (&d)[JB], T (&inv)[JB],
                                              T *sl)
 {
     const int cl = threadIdx.x & 31;
-#pragma unroll
-    for (int c = 0; c < JB; c++) sl[cl * LDB + c] = d[c];
-    wave_sync_lds();
+    (void)sl;
 #pragma unroll
     for (int r = 0; r < JB; r++) {
         // two partial sums halve the dependent chain of the row
         T s0 = (cl == r) ? T(1) : T(0), s1 = T(0);
 #pragma unroll
         for (int l = 0; l + 1 < r; l += 2) {
-            s0 -= sl[r * LDB + l] * inv[l];
-            s1 -= sl[r * LDB + l + 1] * inv[l + 1];
+            s0 -= lane_value(d[l], r) * inv[l];
+            s1 -= lane_value(d[l + 1], r) * inv[l + 1];
         }
-        if (r & 1) s0 -= sl[r * LDB + r - 1] * inv[r - 1];
-        inv[r] = (s0 + s1) * pivot_rcp(sl[r * LDB + r]);
+        if (r & 1) s0 -= lane_value(d[r - 1], r) * inv[r - 1];
+        inv[r] = (s0 + s1) * pivot_rcp(lane_value(d[r], r));
     }
-    wave_sync_lds();
 }
 
 // sD <- the jb x jb diagonal block at (j0, j0) of the tile view L: lower
@@ -684,7 +680,7 @@ int launch_potrf_diag(plasma_enum_t uplo, plasma_desc_t A, int k0, int count,
     // invert-only form (a batch of tiles) 4 per tile
     const void *fn =
         factor ? (const void *)potrf_diag_kernel<T, DiagNT<T>::v,
-                                                 DiagNT<T>::min_blocks>
+                                                 DiagNT<T>::min_waves>
                : (const void *)potrf_diag_kernel<T, NTHREADS>;
     if (lds > 65536) {
         static int attr_set[2][2] = {{0, 0}, {0, 0}};
@@ -697,7 +693,7 @@ int launch_potrf_diag(plasma_enum_t uplo, plasma_desc_t A, int k0, int count,
     }
     if (factor)
         hipLaunchKernelGGL((potrf_diag_kernel<T, DiagNT<T>::v,
-                                              DiagNT<T>::min_blocks>),
+                                              DiagNT<T>::min_waves>),
                            dim3(count),
                            dim3(DiagNT<T>::v), lds, s, A, uplo == PlasmaUpper, k0,
                            factor, (T *)inv, status);

@@ -566,7 +566,7 @@ def split_roofline(r, split):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not "
+        print("bench.py: this benchmark runs on one GPU; --gpus %d is not "
               "supported (multi-GPU sharding is design-only)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)

@@ -7,7 +7,7 @@ test_oracle.py):
   * potrf: ||L - L_oracle||_F / ||L_oracle||_F <= 1e-12, and the entries the
     factorization must not touch are bit-identical to the input;
   * posv / potrs: scaled residual ||B - AX||_inf / (n ||A||_inf ||X||_inf)
-    < 10 * eps64 (the north-star bound, stricter than the tester's 50 eps);
+    < 10 * eps64 (stricter than the tester's 50 eps);
   * a non positive definite input returns LAPACK's info and leaves pA as
     it was (the reference skips its desc2tr copy on a failed sequence).
 """

@@ -18,8 +18,8 @@
 //    GEMM) P(t) = Lbar(k+1+t, k) W_kk; gram_partial_kernel + gram_reduce_
 //    kernel Lbar(k,k) -= sum_t P(t)^T L(k+1+t, k), split over the tiles t
 //    into partial sums added in a fixed order; copy_panel_kernel P ->
-//    Lbar(k+1.., k); TileXW (block GEMM, twice) Lbar(k,k) = W_kk^T Q W_kk
-//    for the diagonal tile's own adjoint.
+//    Lbar(k+1.., k); tile_xw_kernel (twice) Lbar(k,k) = W_kk^T Q W_kk for
+//    the diagonal tile's own adjoint.
 #include "common.hip.h"
 #include "tilegemm.hip.h"
 #include "plasma_kernels.h"
@@ -337,34 +337,37 @@ __global__ __launch_bounds__(NTHREADS) void copy_panel_kernel(
     }
 }
 
-// C = op(X) W on one kv x kv tile (op(X) = X, or X^T with XT), W the
-// inverse of a diagonal tile (column major, ld ldw).
+// C = X W^T-view on one kv x kv tile, 64 x 64 blocks (blockIdx.x): C(i, j)
+// = sum_kk X(i, kk) W(kk, j), X given by its strides (Q, or T1 read
+// transposed), W the inverse of a diagonal tile (column major, ld ldw).
 template <typename T>
-struct TileXW {
-    View<T> X, C;
-    const T *W;
-    int ldw, kv, bpt;
-    __device__ bool operator()(int b, tg::Job<T> &j) const
-    {
-        const int r0 = (b / bpt) * tg::BM, c0 = (b % bpt) * tg::BM;
-        if (r0 >= kv || c0 >= kv) return false;
-        j.X.p = X.p + (ptrdiff_t)r0 * X.rs;
-        j.X.rs = X.rs;
-        j.X.cs = X.cs;
-        j.Y.p = const_cast<T *>(W) + (size_t)c0 * ldw;
-        j.Y.rs = ldw;
-        j.Y.cs = 1;
-        j.C.p = C.p + (ptrdiff_t)r0 * C.rs + (ptrdiff_t)c0 * C.cs;
-        j.C.rs = C.rs;
-        j.C.cs = C.cs;
-        j.xrows = kv - r0;
-        j.yrows = kv - c0;
-        j.K = kv;
-        j.lower = false;
-        j.diag_off = 0;
-        return true;
-    }
-};
+__global__ __launch_bounds__(NTHREADS) void tile_xw_kernel(
+    View<T> X, View<T> C, const T *W, int ldw, int kv, const int *status)
+{
+    typedef GemmNT<T, AB, AB, AKC, 32, 32> G;
+    __shared__ typename G::Smem sm;
+    if (block_should_stop(status)) return;
+    const int bpt = (kv + AB - 1) / AB;
+    const int i0 = (blockIdx.x / bpt) * AB, j0 = (blockIdx.x % bpt) * AB;
+    View<T> Xb = X;
+    Xb.p += (ptrdiff_t)i0 * X.rs;
+    View<T> Y;                          // Y(j, kk) = W(kk, j0 + j)
+    Y.p = const_cast<T *>(W) + (size_t)j0 * ldw;
+    Y.rs = ldw;
+    Y.cs = 1;
+    typename G::acc_t acc[G::FM][G::FN];
+    G::zero(acc);
+    G::run(acc, Xb, kv - i0, Y, kv - j0, kv, sm);
+#pragma unroll
+    for (int i = 0; i < G::FM; i++)
+#pragma unroll
+        for (int j = 0; j < G::FN; j++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int r = i0 + G::acc_row(i, q), c = j0 + G::acc_col(j);
+                if (r < kv && c < kv) C.at(r, c) = acc[i][j][q];
+            }
+}
 
 // Lb(k,k) = W^T Q W = (Q W)^T W (Q symmetric): T1 = Q W, then T1^T W.
 template <typename T>
@@ -373,35 +376,25 @@ int launch_adjoint_diag_sandwich(plasma_desc_t Lb, int k, const T *W,
                                  hipStream_t s)
 {
     const int kv = plasma_tile_nview(Lb, k);
-    const int bpt = (kv + tg::BM - 1) / tg::BM;
-    const int njobs = bpt * bpt;
+    const int bpt = (kv + AB - 1) / AB;
     const int ld = Lb.mb;
-    TileXW<T> d1;
-    d1.X.p = const_cast<T *>(Q);
-    d1.X.rs = 1;
-    d1.X.cs = ld;
-    d1.C.p = T1;
-    d1.C.rs = 1;
-    d1.C.cs = ld;
-    d1.W = W;
-    d1.ldw = Lb.mb;
-    d1.kv = kv;
-    d1.bpt = bpt;
-    hipLaunchKernelGGL((tg::kernel<T, true, false, false, true, false,
-                                   TileXW<T>>),
-                       dim3(tg::xcd_grid(njobs)), dim3(tg::NT), 0, s, d1,
-                       njobs, status);
-    TileXW<T> d2 = d1;
-    d2.X.p = T1;                    // X(r, kk) = T1(kk, r)
-    d2.X.rs = ld;
-    d2.X.cs = 1;
-    d2.C.p = (T *)plasma_tile_addr(Lb, k, k);
-    d2.C.rs = 1;
-    d2.C.cs = plasma_tile_mmain(Lb, k);
-    hipLaunchKernelGGL((tg::kernel<T, true, false, false, false, false,
-                                   TileXW<T>>),
-                       dim3(tg::xcd_grid(njobs)), dim3(tg::NT), 0, s, d2,
-                       njobs, status);
+    View<T> X, C;
+    X.p = const_cast<T *>(Q);           // X(i, kk) = Q(i, kk)
+    X.rs = 1;
+    X.cs = ld;
+    C.p = T1;
+    C.rs = 1;
+    C.cs = ld;
+    hipLaunchKernelGGL(tile_xw_kernel<T>, dim3(bpt * bpt), dim3(NTHREADS), 0,
+                       s, X, C, W, ld, kv, status);
+    X.p = T1;                           // X(i, kk) = T1(kk, i)
+    X.rs = ld;
+    X.cs = 1;
+    C.p = (T *)plasma_tile_addr(Lb, k, k);
+    C.rs = 1;
+    C.cs = plasma_tile_mmain(Lb, k);
+    hipLaunchKernelGGL(tile_xw_kernel<T>, dim3(bpt * bpt), dim3(NTHREADS), 0,
+                       s, X, C, W, ld, kv, status);
     return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
 }
 

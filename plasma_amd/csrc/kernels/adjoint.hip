@@ -14,14 +14,13 @@
 //    move in contiguous runs.
 //  * syr2k_lapack_kernel: C -= s (U V^T + V U^T) on LAPACK-layout arrays
 //    (the potrs adjoint's symmetric rank-2k term, nrhs small).
-//  * Steps of the blocked reverse sweep (lower storage): PanelW (block
-//    GEMM) P(t) = Lbar(k+1+t, k) W_kk; gram_partial_kernel + gram_reduce_
+//  * Steps of the blocked reverse sweep (lower storage): panel_w_kernel
+//    P(t) = Lbar(k+1+t, k) W_kk; gram_partial_kernel + gram_reduce_
 //    kernel Lbar(k,k) -= sum_t P(t)^T L(k+1+t, k), split over the tiles t
 //    into partial sums added in a fixed order; copy_panel_kernel P ->
 //    Lbar(k+1.., k); tile_xw_kernel (twice) Lbar(k,k) = W_kk^T Q W_kk for
 //    the diagonal tile's own adjoint.
 #include "common.hip.h"
-#include "tilegemm.hip.h"
 #include "plasma_kernels.h"
 
 namespace {
@@ -224,39 +223,6 @@ __global__ __launch_bounds__(NTHREADS) void syr2k_lapack_kernel(
     C[i + (size_t)j * ldc] -= s * acc;
 }
 
-// P(t) = Lbar(k+1+t, k) W_kk: jobs (t, 128-row block, 128-column block).
-template <typename T>
-struct PanelW {
-    plasma_desc_t Lb;     // general, lower tiles hold Lbar
-    int k, bpt;
-    const T *W;           // W_kk, column major, ld mb
-    T *P;                 // panel scratch, tile t at P + t mb nb, ld mb
-    __device__ bool operator()(int b, tg::Job<T> &j) const
-    {
-        const int t = b / (bpt * bpt), sub_ = b % (bpt * bpt);
-        const int r0 = (sub_ / bpt) * tg::BM, c0 = (sub_ % bpt) * tg::BM;
-        const int mv = plasma_tile_mview(Lb, k + 1 + t);
-        const int kv = plasma_tile_nview(Lb, k);
-        if (r0 >= mv || c0 >= kv) return false;
-        const View<T> X = tile_view<T>(Lb, k + 1 + t, k);
-        j.X.p = X.p + r0;
-        j.X.rs = 1;
-        j.X.cs = X.cs;
-        j.Y.p = const_cast<T *>(W) + (size_t)c0 * Lb.mb;  // Y(c, kk) = W(kk, c)
-        j.Y.rs = Lb.mb;
-        j.Y.cs = 1;
-        j.C.p = P + (size_t)t * Lb.mb * Lb.nb + r0 + (size_t)c0 * Lb.mb;
-        j.C.rs = 1;
-        j.C.cs = Lb.mb;
-        j.xrows = mv - r0;
-        j.yrows = kv - c0;
-        j.K = kv;
-        j.lower = false;
-        j.diag_off = 0;
-        return true;
-    }
-};
-
 // part[z](i, j) = sum over tiles t of split z of P(t)^T L(k+1+t, k), 64 x 64
 // blocks of the nb x nb result (block (bi, bj) = blockIdx.x).
 template <typename T>
@@ -337,6 +303,33 @@ __global__ __launch_bounds__(NTHREADS) void copy_panel_kernel(
     }
 }
 
+// One 64 x 64 block (i0, j0) of C = X W: C(i, j) = sum_kk X(i, kk) W(kk, j)
+// over kk < kv, for i < m rows and j < kv columns.
+template <typename T, typename G>
+__device__ __forceinline__ void xw_block(View<T> X, const View<T> &C,
+                                         const T *W, int ldw, int m, int kv,
+                                         int i0, int j0,
+                                         typename G::Smem &sm)
+{
+    X.p += (ptrdiff_t)i0 * X.rs;
+    View<T> Y;                          // Y(j, kk) = W(kk, j0 + j)
+    Y.p = const_cast<T *>(W) + (size_t)j0 * ldw;
+    Y.rs = ldw;
+    Y.cs = 1;
+    typename G::acc_t acc[G::FM][G::FN];
+    G::zero(acc);
+    G::run(acc, X, m - i0, Y, kv - j0, kv, sm);
+#pragma unroll
+    for (int i = 0; i < G::FM; i++)
+#pragma unroll
+        for (int j = 0; j < G::FN; j++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int r = i0 + G::acc_row(i, q), c = j0 + G::acc_col(j);
+                if (r < m && c < kv) C.at(r, c) = acc[i][j][q];
+            }
+}
+
 // C = X W^T-view on one kv x kv tile, 64 x 64 blocks (blockIdx.x): C(i, j)
 // = sum_kk X(i, kk) W(kk, j), X given by its strides (Q, or T1 read
 // transposed), W the inverse of a diagonal tile (column major, ld ldw).
@@ -349,25 +342,32 @@ __global__ __launch_bounds__(NTHREADS) void tile_xw_kernel(
     if (block_should_stop(status)) return;
     const int bpt = (kv + AB - 1) / AB;
     const int i0 = (blockIdx.x / bpt) * AB, j0 = (blockIdx.x % bpt) * AB;
-    View<T> Xb = X;
-    Xb.p += (ptrdiff_t)i0 * X.rs;
-    View<T> Y;                          // Y(j, kk) = W(kk, j0 + j)
-    Y.p = const_cast<T *>(W) + (size_t)j0 * ldw;
-    Y.rs = ldw;
-    Y.cs = 1;
-    typename G::acc_t acc[G::FM][G::FN];
-    G::zero(acc);
-    G::run(acc, Xb, kv - i0, Y, kv - j0, kv, sm);
-#pragma unroll
-    for (int i = 0; i < G::FM; i++)
-#pragma unroll
-        for (int j = 0; j < G::FN; j++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int r = i0 + G::acc_row(i, q), c = j0 + G::acc_col(j);
-                if (r < kv && c < kv) C.at(r, c) = acc[i][j][q];
-            }
+    xw_block<T, G>(X, C, W, ldw, kv, kv, i0, j0, sm);
 }
+
+// P(t) = Lbar(k+1+t, k) W_kk for the panel below tile k, 64 x 64 blocks:
+// blockIdx.x the block of the tile, blockIdx.y the tile t.
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void panel_w_kernel(
+    plasma_desc_t Lb, int k, const T *W, T *P, const int *status)
+{
+    typedef GemmNT<T, AB, AB, AKC, 32, 32> G;
+    __shared__ typename G::Smem sm;
+    if (block_should_stop(status)) return;
+    const int t = blockIdx.y;
+    const int mv = plasma_tile_mview(Lb, k + 1 + t);
+    const int kv = plasma_tile_nview(Lb, k);
+    const int bpc = (Lb.nb + AB - 1) / AB;      // column blocks per tile
+    const int i0 = (blockIdx.x / bpc) * AB, j0 = (blockIdx.x % bpc) * AB;
+    if (i0 >= mv || j0 >= kv) return;
+    const View<T> X = tile_view<T>(Lb, k + 1 + t, k);
+    View<T> C;
+    C.p = P + (size_t)t * Lb.mb * Lb.nb;
+    C.rs = 1;
+    C.cs = Lb.mb;
+    xw_block<T, G>(X, C, W, Lb.mb, mv, kv, i0, j0, sm);
+}
+
 
 // Lb(k,k) = W^T Q W = (Q W)^T W (Q symmetric): T1 = Q W, then T1^T W.
 template <typename T>
@@ -406,13 +406,9 @@ int launch_adjoint_panel_step(plasma_desc_t L, plasma_desc_t Lb, int k,
     const int ntiles = L.mt - k - 1;
     if (ntiles <= 0) return 0;
     // P = Lbar(k+1.., k) W_kk
-    const int bpt = (L.mb + tg::BM - 1) / tg::BM;
-    PanelW<T> d{Lb, k, bpt, W, P};
-    const int njobs = ntiles * bpt * bpt;
-    hipLaunchKernelGGL((tg::kernel<T, true, false, false, true, false,
-                                   PanelW<T>>),
-                       dim3(tg::xcd_grid(njobs)), dim3(tg::NT), 0, s, d,
-                       njobs, status);
+    const int bpc = (L.nb + AB - 1) / AB, bpr = (L.mb + AB - 1) / AB;
+    hipLaunchKernelGGL(panel_w_kernel<T>, dim3(bpr * bpc, ntiles),
+                       dim3(NTHREADS), 0, s, Lb, k, W, P, status);
     // Lbar(k,k) -= sum_t P(t)^T L(k+1+t, k)
     const int gb = (L.nb + AB - 1) / AB;
     const int nsplit = max(1, min(ntiles, maxsplit));

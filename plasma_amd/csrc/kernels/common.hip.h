@@ -142,9 +142,13 @@ struct GemmNT {
     static_assert((BM / WM) * (BN / WN) == 4, "four waves");
     static_assert(LX >= 1 && LY >= 1, "chunk too small");
 
+    // LDS rows padded by 16 bytes: row starts stay 16-byte aligned for the
+    // vector stores, and the transposed stores (one lane per k value, W
+    // rows apart) no longer land in one bank
+    static constexpr int PAD = 16 / (int)sizeof(T);
     struct Smem {
-        T x[2][KC][BM];
-        T y[2][KC][BN];
+        T x[2][KC][BM + PAD];
+        T y[2][KC][BN + PAD];
     };
 
     // Chunk loads into registers, W elements per 16-byte vector.  With
@@ -223,9 +227,9 @@ struct GemmNT {
     }
 
     // Registers -> LDS chunk s[kk][row], with the mapping load() used.
-    template <int ROWS, int L>
+    template <int ROWS, int L, int LD>
     static __device__ __forceinline__ void store(const T (&reg)[L],
-                                                 T (*s)[ROWS], bool rm)
+                                                 T (*s)[LD], bool rm)
     {
         const int t = threadIdx.x;
         if (rm) {

@@ -156,8 +156,9 @@ __global__ __launch_bounds__(NTHREADS) void amax_cols_kernel(
 // R(m, tn) -= sum_k Asym(m, k) X(k, tn) with Asym(m,k) = L(m,k) (k < m),
 // L(k,m)^T (k > m), and the diagonal tile read symmetrically from its
 // stored triangle.  64 x 64 output blocks; K runs over every tile column in
-// 16-deep chunks through double-buffered LDS with 16-byte loads (32 KB of
-// LDS, so two workgroups share a CU and hide each other's loads); MFMA
+// 16-deep chunks through double-buffered LDS with 16-byte loads (33 KB of
+// LDS with the padded rows: up to four workgroups share a CU and hide each
+// other's loads); MFMA
 // accumulation in the precision of A.  The product is formed transposed
 // (X^T Asym^T) so the accumulator's consecutive lanes are consecutive rows
 // of R.

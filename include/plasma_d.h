@@ -62,6 +62,33 @@ void plasma_omp_dtr2desc(double *pA, int lda, plasma_desc_t A,
 void plasma_omp_ddesc2tr(plasma_desc_t A, double *pA, int lda,
                          plasma_sequence_t *sequence, plasma_request_t *request);
 
+/*
+ * Norms (reference compute/zlansy.c:81, :240; compute/zlange.c:76, :233;
+ * compute/dzamax.c:21, :113).  norm: PlasmaMaxNorm, PlasmaOneNorm,
+ * PlasmaInfNorm or PlasmaFrobeniusNorm; colrow: PlasmaColumnwise (values
+ * has n entries) or PlasmaRowwise (m entries).  The synchronous forms
+ * return the norm, or a negative argument code.  Tile forms: work is device
+ * memory of plasma_norm_work_doubles(A) doubles or NULL (library
+ * workspace); value / values may be host or device memory and hold the
+ * result after plasma_sequence_sync; views start on a tile boundary.
+ */
+double plasma_dlansy(plasma_enum_t norm, plasma_enum_t uplo, int n,
+                     double *pA, int lda);
+double plasma_dlange(plasma_enum_t norm, int m, int n, double *pA, int lda);
+int plasma_damax(plasma_enum_t colrow, int m, int n, double *pA, int lda,
+                 double *values);
+void plasma_omp_dlansy(plasma_enum_t norm, plasma_enum_t uplo,
+                       plasma_desc_t A, double *work, double *value,
+                       plasma_sequence_t *sequence, plasma_request_t *request);
+void plasma_omp_dlange(plasma_enum_t norm, plasma_desc_t A, double *work,
+                       double *value, plasma_sequence_t *sequence,
+                       plasma_request_t *request);
+void plasma_omp_damax(plasma_enum_t colrow, plasma_desc_t A, double *work,
+                      double *values, plasma_sequence_t *sequence,
+                      plasma_request_t *request);
+/* No reference counterpart: device workspace (doubles) of the tile norms. */
+size_t plasma_norm_work_doubles(plasma_desc_t A);
+
 #ifdef __cplusplus
 }
 #endif

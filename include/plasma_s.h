@@ -62,6 +62,23 @@ void plasma_omp_str2desc(float *pA, int lda, plasma_desc_t A,
 void plasma_omp_sdesc2tr(plasma_desc_t A, float *pA, int lda,
                          plasma_sequence_t *sequence, plasma_request_t *request);
 
+/* Norms (reference compute/zlansy.c:81, :240; compute/zlange.c:76, :233;
+ * compute/dzamax.c:21, :113), single precision forms of plasma_d.h's. */
+float plasma_slansy(plasma_enum_t norm, plasma_enum_t uplo, int n,
+                    float *pA, int lda);
+float plasma_slange(plasma_enum_t norm, int m, int n, float *pA, int lda);
+int plasma_samax(plasma_enum_t colrow, int m, int n, float *pA, int lda,
+                 float *values);
+void plasma_omp_slansy(plasma_enum_t norm, plasma_enum_t uplo,
+                       plasma_desc_t A, double *work, float *value,
+                       plasma_sequence_t *sequence, plasma_request_t *request);
+void plasma_omp_slange(plasma_enum_t norm, plasma_desc_t A, double *work,
+                       float *value, plasma_sequence_t *sequence,
+                       plasma_request_t *request);
+void plasma_omp_samax(plasma_enum_t colrow, plasma_desc_t A, double *work,
+                      float *values, plasma_sequence_t *sequence,
+                      plasma_request_t *request);
+
 #ifdef __cplusplus
 }
 #endif

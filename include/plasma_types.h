@@ -117,8 +117,26 @@ enum {
     PlasmaGraphCapture     = 1002, /* PlasmaEnabled: replay hipGraphs */
     PlasmaDevice           = 1003, /* HIP device ordinal (read only) */
     PlasmaUpdateDepth      = 1004, /* tile columns per trailing update, 1..4 */
-    PlasmaSplitUpdate      = 1005, /* FP32 updates on bf16 MFMA (split terms) */
+    PlasmaSplitUpdate      = 1005, /* FP32 updates on 16-bit MFMA (below) */
+    PlasmaHalfInput        = 1006, /* input format of plasma_dhposv's factor */
     PlasmaParamUnknown     = INT_MAX
+};
+
+/*
+ * Values of PlasmaSplitUpdate (extension).  FP32 trailing updates run on the
+ * 16-bit matrix cores from exact splits of the panel entries:
+ *   PlasmaSplitFp16: rows scaled by powers of two so every factor entry is
+ *                    at most 1 in magnitude, two fp16 terms, three products
+ *                    (22-bit operands, FP32 accumulation);
+ *   PlasmaSplitBf16: three bf16 terms, six products (FP32-exact products);
+ *   PlasmaEnabled:   the default split (PlasmaSplitBf16);
+ *   PlasmaDisabled:  FP32 MFMA.
+ * Values of PlasmaHalfInput: PlasmaSplitBf16 (default) or PlasmaSplitFp16,
+ * the one-term input format of plasma_dhposv's low-precision factor.
+ */
+enum {
+    PlasmaSplitBf16 = 2,
+    PlasmaSplitFp16 = 3
 };
 
 typedef int plasma_enum_t;

@@ -9,6 +9,12 @@ import subprocess
 
 import numpy as np
 
+# MKL's tile kernels run sequentially inside the oracle's OpenMP tasks.
+# MKL fixes its threading layer at its first routine call in the process,
+# so the choice is also made through the environment, before anything
+# (a test helper, a generator) can call MKL first.
+os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 LOWER, UPPER = 122, 121
@@ -32,6 +38,10 @@ def load():
         pi = ctypes.POINTER(ctypes.c_int)
         sig = {
             "ora_set_blas": (i, [i]), "ora_get_blas": (i, []),
+            "ora_mkl_refused": (i, []),
+            "ora_dlansy": (d, [i, i, i, p, i]),
+            "ora_dlange": (d, [i, i, i, p, i]),
+            "ora_damax": (None, [i, i, i, p, i, p]),
             "ora_set_threads": (None, [i]), "ora_get_threads": (i, []),
             "ora_plant_mistake": (None, [i]),
             "ora_dpotrf": (i, [i, i, p, i, i]),
@@ -67,7 +77,15 @@ def _ld(a):
 
 
 def set_blas(backend):
-    return load().ora_set_blas(backend)
+    """Select the tile-kernel backend; returns the one in force (BUILTIN
+    when MKL is absent).  Raises when MKL is present but running on a
+    threaded layer, whose results are wrong in a process that also holds
+    torch's libgomp."""
+    got = load().ora_set_blas(backend)
+    if backend == MKL and got != MKL and load().ora_mkl_refused():
+        raise RuntimeError("oracle: MKL is not on its sequential threading "
+                           "layer (an MKL routine ran first)")
+    return got
 
 
 def set_threads(n):
@@ -149,6 +167,29 @@ def tile2lapack(uplo, tiles, m, n, nb, init=None):
     t = np.ascontiguousarray(tiles, dtype=np.float64)
     load().ora_tile2lapack(uplo, m, n, t.ctypes.data, a.ctypes.data, _ld(a), nb)
     return a
+
+
+MAXNORM, ONENORM, INFNORM, FRONORM = 177, 171, 175, 174
+COLUMNWISE, ROWWISE = 401, 402
+
+
+def lansy(norm, uplo, a):
+    a = _f(a, np.float64)
+    return load().ora_dlansy(norm, uplo, a.shape[0], a.ctypes.data, _ld(a))
+
+
+def lange(norm, a):
+    a = _f(a, np.float64)
+    return load().ora_dlange(norm, a.shape[0], a.shape[1], a.ctypes.data,
+                             _ld(a))
+
+
+def amax(colrow, a):
+    a = _f(a, np.float64)
+    m, n = a.shape
+    v = np.zeros(n if colrow == COLUMNWISE else m)
+    load().ora_damax(colrow, m, n, a.ctypes.data, _ld(a), v.ctypes.data)
+    return v
 
 
 def plant_mistake(which):

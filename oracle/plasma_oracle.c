@@ -17,6 +17,7 @@
 #include <math.h>
 #include <omp.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -25,6 +26,7 @@
 enum { ORA_NOTRANS = 111, ORA_TRANS = 112, ORA_LEFT = 141, ORA_RIGHT = 142 };
 
 static int ora_backend = ORA_BLAS_BUILTIN;
+static int mkl_layer_refused = 0;
 static int ora_threads = 0;
 static int ora_planted = 0;
 
@@ -84,9 +86,24 @@ static int mkl_load(void)
     for (int i = 0; i < 5 && h == NULL; i++)
         if (names[i] != NULL) h = dlopen(names[i], RTLD_NOW | RTLD_GLOBAL);
     if (h == NULL) return 0;
-    /* tile kernels must be sequential: parallelism comes from the tasks */
+    /* Tile kernels must be sequential: parallelism comes from the tasks.
+     * MKL keeps the layer it started with once any of its routines has
+     * run, and its Intel-OpenMP layer gives wrong results in a process
+     * where a GNU libgomp was loaded globally first (torch does this):
+     * the layer forks teams through libiomp5's __kmpc_fork_call while its
+     * omp_get_thread_num / omp_get_num_threads bind to libgomp's, so every
+     * thread computes the same share.  The MKL backend is refused unless
+     * the sequential layer is the one in force. */
     int (*set_layer)(int) = (int (*)(int))dlsym(h, "MKL_Set_Threading_Layer");
-    if (set_layer) set_layer(1 /* MKL_THREADING_SEQUENTIAL */);
+    int layer = set_layer ? set_layer(1 /* MKL_THREADING_SEQUENTIAL */) : -1;
+    if (layer != 1) {
+        fprintf(stderr, "oracle: MKL threading layer is %d, not sequential "
+                "(an MKL routine ran before the oracle selected its layer; "
+                "set MKL_THREADING_LAYER=SEQUENTIAL); MKL backend refused\n",
+                layer);
+        mkl_layer_refused = 1;
+        return 0;
+    }
     mkl.dgemm = (gemm_d_t)dlsym(h, "cblas_dgemm");
     mkl.dtrsm = (trsm_d_t)dlsym(h, "cblas_dtrsm");
     mkl.dsyrk = (syrk_d_t)dlsym(h, "cblas_dsyrk");
@@ -109,6 +126,9 @@ int ora_set_blas(int backend)
     return ora_backend;
 }
 int ora_get_blas(void) { return ora_backend; }
+/* 1 when MKL was found but refused because its threading layer is not the
+ * sequential one */
+int ora_mkl_refused(void) { return mkl_layer_refused; }
 
 /* ---------------------------------------------------------------------
  * Tile descriptor (reference include/plasma_descriptor.h:40-246,
@@ -330,6 +350,119 @@ static void d_amax_cols(int m, int n, const double *X, int ldx, double *v)
             if (a > t) t = a;
         }
         v[j] = t;
+    }
+}
+
+/* ---------------------------------------------------------------------
+ * Norms and column / row max-norms, the public plasma_dlansy /
+ * plasma_dlange / plasma_damax (reference compute/zlansy.c:81,
+ * compute/zlange.c:76, compute/dzamax.c; tile algorithms pzlansy.c,
+ * pzlange.c, pdzamax.c).  The reference forms per-tile partial results
+ * and combines them; the values here are the same quantities summed in
+ * LAPACK order: Max = max |a_ij|, One = max column sum of |a_ij|, Inf = max
+ * row sum, Frobenius = sqrt(sum a_ij^2) by the scaled sum of squares of
+ * LAPACK's dlassq (no overflow for large entries).  NaN propagates as in
+ * LAPACK's disnan tests.
+ * --------------------------------------------------------------------- */
+enum { ORA_ONE = 171, ORA_FRO = 174, ORA_INF = 175, ORA_MAX = 177,
+       ORA_COLWISE = 401, ORA_ROWWISE = 402 };
+
+static void ssq_add(double x, double *scale, double *ssq)
+{
+    double a = fabs(x);
+    if (a == 0.0) return;
+    if (isnan(a)) { *scale = a; *ssq = 1.0; return; }
+    if (*scale < a) {
+        *ssq = 1.0 + *ssq * (*scale / a) * (*scale / a);
+        *scale = a;
+    }
+    else {
+        *ssq += (a / *scale) * (a / *scale);
+    }
+}
+
+static double ge_elem(int uplo, const double *A, int lda, int i, int j)
+{
+    /* element (i, j) of a symmetric matrix stored in the uplo triangle, or
+     * of a general matrix (uplo = 0) */
+    if ((uplo == ORA_LOWER && i < j) || (uplo == ORA_UPPER && i > j)) {
+        int t = i; i = j; j = t;
+    }
+    return A[(size_t)j * lda + i];
+}
+
+static double norm_any(int norm, int uplo, int m, int n, const double *A,
+                       int lda)
+{
+    if (m == 0 || n == 0) return 0.0;
+    double value = 0.0;
+    if (norm == ORA_MAX) {
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < m; i++) {
+                double a = fabs(ge_elem(uplo, A, lda, i, j));
+                if (a > value || isnan(a)) value = a;
+            }
+    }
+    else if (norm == ORA_ONE) {
+        for (int j = 0; j < n; j++) {
+            double s = 0;
+            for (int i = 0; i < m; i++) s += fabs(ge_elem(uplo, A, lda, i, j));
+            if (s > value || isnan(s)) value = s;
+        }
+    }
+    else if (norm == ORA_INF) {
+        for (int i = 0; i < m; i++) {
+            double s = 0;
+            for (int j = 0; j < n; j++) s += fabs(ge_elem(uplo, A, lda, i, j));
+            if (s > value || isnan(s)) value = s;
+        }
+    }
+    else if (norm == ORA_FRO) {
+        double scale = 0.0, ssq = 1.0;
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < m; i++)
+                ssq_add(ge_elem(uplo, A, lda, i, j), &scale, &ssq);
+        value = scale * sqrt(ssq);
+    }
+    else {
+        value = -1.0;
+    }
+    return value;
+}
+
+double ora_dlansy(int norm, int uplo, int n, const double *A, int lda)
+{
+    return norm_any(norm, uplo, n, n, A, lda);
+}
+
+double ora_dlange(int norm, int m, int n, const double *A, int lda)
+{
+    return norm_any(norm, 0, m, n, A, lda);
+}
+
+/* values[j] = max_i |a_ij| (Columnwise) or values[i] = max_j |a_ij|
+ * (Rowwise); reference compute/pdzamax.c with core_dzamax.c, where a NaN
+ * entry does not replace a larger value (plain > comparison). */
+void ora_damax(int colrow, int m, int n, const double *A, int lda,
+               double *values)
+{
+    if (colrow == ORA_COLWISE) {
+        for (int j = 0; j < n; j++) {
+            double t = 0.0;
+            for (int i = 0; i < m; i++) {
+                double a = fabs(A[(size_t)j * lda + i]);
+                if (a > t) t = a;
+            }
+            values[j] = t;
+        }
+    }
+    else {
+        for (int i = 0; i < m; i++) values[i] = 0.0;
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < m; i++) {
+                double a = fabs(A[(size_t)j * lda + i]);
+                if (a > values[i]) values[i] = a;
+            }
     }
 }
 

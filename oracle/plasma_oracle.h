@@ -32,6 +32,12 @@ enum { ORA_UPPER = 121, ORA_LOWER = 122 };
  * back to built-in when it cannot be loaded). */
 int ora_set_blas(int backend);
 int ora_get_blas(void);
+int ora_mkl_refused(void);
+/* norms (reference compute/zlansy.c, zlange.c, dzamax.c) */
+double ora_dlansy(int norm, int uplo, int n, const double *A, int lda);
+double ora_dlange(int norm, int m, int n, const double *A, int lda);
+void ora_damax(int colrow, int m, int n, const double *A, int lda,
+               double *values);
 /* Threads used by the OpenMP task runtime (0 = keep the default). */
 void ora_set_threads(int nthreads);
 int ora_get_threads(void);

@@ -27,7 +27,9 @@ PlasmaUpper, PlasmaLower, PlasmaGeneral = 121, 122, 123
 PlasmaNonUnit, PlasmaUnit = 131, 132
 PlasmaLeft, PlasmaRight = 141, 142
 PlasmaInfNorm = 175
-PlasmaColumnwise = 401
+PlasmaColumnwise, PlasmaRowwise = 401, 402
+PlasmaOneNorm, PlasmaFrobeniusNorm, PlasmaInfNorm, PlasmaMaxNorm = (171, 174,
+                                                                    175, 177)
 PlasmaSuccess = 0
 PlasmaErrorNotInitialized = -101
 PlasmaErrorNotSupported = -102
@@ -40,6 +42,8 @@ PlasmaTuning, PlasmaNb, PlasmaIb = 0, 1, 2
 PlasmaLookahead, PlasmaGraphCapture, PlasmaDevice = 1001, 1002, 1003
 PlasmaUpdateDepth = 1004
 PlasmaSplitUpdate = 1005
+PlasmaHalfInput = 1006
+PlasmaSplitBf16, PlasmaSplitFp16 = 2, 3
 
 c_int, c_void_p, c_size_t, c_double = (ctypes.c_int, ctypes.c_void_p,
                                        ctypes.c_size_t, ctypes.c_double)
@@ -107,7 +111,20 @@ for _p, _t in (("d", "double"), ("s", "float")):
         "plasma_omp_%sdesc2ge" % _p: (None, [_D, c_void_p, c_int, _PS, _PR]),
         "plasma_omp_%str2desc" % _p: (None, [c_void_p, c_int, _D, _PS, _PR]),
         "plasma_omp_%sdesc2tr" % _p: (None, [_D, c_void_p, c_int, _PS, _PR]),
+        "plasma_%slansy" % _p: (c_double if _p == "d" else ctypes.c_float,
+                                [c_int, c_int, c_int, c_void_p, c_int]),
+        "plasma_%slange" % _p: (c_double if _p == "d" else ctypes.c_float,
+                                [c_int, c_int, c_int, c_void_p, c_int]),
+        "plasma_%samax" % _p: (c_int, [c_int, c_int, c_int, c_void_p, c_int,
+                                       c_void_p]),
+        "plasma_omp_%slansy" % _p: (None, [c_int, c_int, _D, c_void_p,
+                                           c_void_p, _PS, _PR]),
+        "plasma_omp_%slange" % _p: (None, [c_int, _D, c_void_p, c_void_p,
+                                           _PS, _PR]),
+        "plasma_omp_%samax" % _p: (None, [c_int, _D, c_void_p, c_void_p,
+                                          _PS, _PR]),
     })
+PROTOTYPES["plasma_norm_work_doubles"] = (c_size_t, [_D])
 PROTOTYPES.update({
     "plasma_dsposv": (c_int, [c_int, c_int, c_int, c_void_p, c_int, c_void_p,
                               c_int, c_void_p, c_int, _PI]),

@@ -30,9 +30,14 @@ static int by_n_desc(const void *a, const void *b)
     return x->i - y->i;
 }
 
-/* device copy of a host or device array (owned copy in *dev_out) */
+/* Device copy of a host or device array (owned copy in *dev_out).  The
+ * copy goes on the context stream: pool memory is handed out in that
+ * stream's order, and a block returned by an earlier call may still be read
+ * by its queued kernels, so a copy on another stream (or the null stream,
+ * which the library's non-blocking streams do not wait for) could overwrite
+ * it under them. */
 static int stage_in(const void *arr, size_t bytes, void **dev_out,
-                    int *owned)
+                    int *owned, hipStream_t s)
 {
     if (plasma_is_device_ptr(arr)) {
         *dev_out = (void *)arr;
@@ -43,8 +48,9 @@ static int stage_in(const void *arr, size_t bytes, void **dev_out,
     if (*dev_out == NULL)
         return PlasmaErrorOutOfMemory;
     *owned = 1;
-    return plasma_hip_check(hipMemcpy(*dev_out, arr, bytes,
-                                      hipMemcpyHostToDevice), "hipMemcpy")
+    return plasma_hip_check(hipMemcpyAsync(*dev_out, arr, bytes,
+                                           hipMemcpyHostToDevice, s),
+                            "hipMemcpyAsync")
                ? PlasmaErrorInternal : 0;
 }
 
@@ -111,6 +117,13 @@ static int vbatched(plasma_enum_t precision, plasma_enum_t uplo, int batch,
         retval = PlasmaErrorOutOfMemory;
         goto done;
     }
+    /* a caller's device arrays may have been written by work queued on the
+     * library's streams (non-blocking, so the copies below would not wait
+     * for it) */
+    if (plasma_is_device_ptr(n) || plasma_is_device_ptr(aoff)) {
+        hipStreamSynchronize(ctx->panel_stream);
+        hipStreamSynchronize(ctx->stream);
+    }
     if (plasma_hip_check(hipMemcpy(hn, n, nbytes, hipMemcpyDefault),
                          "hipMemcpy(n)") ||
         plasma_hip_check(hipMemcpy(hoff, aoff, obytes, hipMemcpyDefault),
@@ -143,10 +156,11 @@ static int vbatched(plasma_enum_t precision, plasma_enum_t uplo, int batch,
         retval = PlasmaErrorOutOfMemory;
         goto done;
     }
-    if ((retval = stage_in(n, nbytes, &dn, &own_n)) != 0 ||
-        (retval = stage_in(aoff, obytes, &doff, &own_off)) != 0 ||
+    if ((retval = stage_in(n, nbytes, &dn, &own_n, ctx->stream)) != 0 ||
+        (retval = stage_in(aoff, obytes, &doff, &own_off, ctx->stream)) != 0 ||
         (nrhs > 0 &&
-         (retval = stage_in(boff, obytes, &dboff, &own_boff)) != 0))
+         (retval = stage_in(boff, obytes, &dboff, &own_boff, ctx->stream)) !=
+             0))
         goto done;
     own_info = !plasma_is_device_ptr(info);
     if (own_info) {
@@ -192,6 +206,9 @@ static int vbatched(plasma_enum_t precision, plasma_enum_t uplo, int batch,
         retval = PlasmaErrorInternal;
 
 done:
+    /* the host arrays below are freed only after the copies that read them
+     * (also on the error paths) */
+    hipStreamSynchronize(ctx->stream);
     if (own_n) plasma_dev_free(dn);
     if (own_off) plasma_dev_free(doff);
     if (own_boff) plasma_dev_free(dboff);

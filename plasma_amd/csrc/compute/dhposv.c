@@ -420,9 +420,10 @@ int plasma_dhposv(plasma_enum_t uplo, int n, int nrhs, double *pA, int lda,
 
     /* the bf16-input factor; a breakdown clears the status word and falls
      * back to FP64 */
-    ctx->bf16_factor = 1;
+    ctx->low_factor = ctx->half_input == PlasmaSplitBf16 ? PlasmaLowBf16
+                                                         : PlasmaLowFp16;
     plasma_pxpotrf(uplo, As, d.inv_s, &sequence, &request);
-    ctx->bf16_factor = 0;
+    ctx->low_factor = 0;
     if (sequence.status != PlasmaSuccess ||
         read_word(ctx, d.st, &word, &sequence, &request))
         goto done;
@@ -477,7 +478,7 @@ int plasma_dhposv(plasma_enum_t uplo, int n, int nrhs, double *pA, int lda,
 done:
     plasma_pxdesc2lapack(PlasmaGeneral, X, pX, ldx, &sequence, &request);
     int status = plasma_sequence_sync(&sequence);
-    ctx->bf16_factor = 0;
+    ctx->low_factor = 0;
     plasma_dev_free(inv_d);
     plasma_dev_free(d.mpan);
     plasma_dev_free(d.dots);

@@ -15,17 +15,15 @@
  * first updated by the group's earlier panels), and the trailing matrix is
  * then updated once by all g panels together, a product with K = g * nb
  * instead of g products with K = nb (the MFMA update is 15-20 % faster at
- * K = 512 than at K = 256 on gfx950).  With lookahead the update of group q
- * is split: the columns of group q+1 are updated first on the main stream,
- * the panel stream then factors group q+1 while the main stream updates
- * the remaining columns.  The panel solves write into a panel buffer that
+ * K = 512 than at K = 256 on gfx950).  With lookahead depth d the panel
+ * stream runs up to d groups ahead: it applies the d previous groups'
+ * panels to a group's columns itself and factors the group, while the main
+ * stream's bulk update of each group covers only the columns beyond that
+ * window (pxpotrf_groups).  The panel solves write into panel buffers that
  * the updates read; the copy into A rides along with the bulk update, off
- * the critical path.
- * Events carry the two dependencies the depend clauses expressed:
- *     panel(k+1) after update(k, k+1)           (ev_col)
- *     update(k+1, ...) after panel(k+1)         (ev_panel)
- * so the critical path potrf -> trsm -> next column overlaps the bulk
- * update.  The streams are joined again before returning.
+ * the critical path.  Events carry the dependencies the depend clauses
+ * expressed, so the critical path potrf -> trsm -> next column overlaps the
+ * bulk updates.  The streams are joined again before returning.
  */
 #include <string.h>
 
@@ -82,6 +80,20 @@ static int k_potrf(plasma_enum_t uplo, plasma_desc_t A, int k, void *inv,
     return err;
 }
 
+/* How the trailing updates of an FP32 factorization run (use_split). */
+enum {
+    SPLIT_NONE = 0,     /* FP32 MFMA */
+    SPLIT_BF16X3,       /* three bf16 terms, six products (FP32-exact) */
+    SPLIT_BF16X1,       /* bf16 inputs, one product */
+    SPLIT_FP16X2,       /* row-scaled two-term fp16, three products */
+    SPLIT_FP16X1        /* row-scaled fp16 inputs, one product */
+};
+
+static int split_is_fp16(int split)
+{
+    return split == SPLIT_FP16X2 || split == SPLIT_FP16X1;
+}
+
 /* the column-group context of the launches below */
 typedef struct {
     plasma_enum_t uplo;
@@ -89,22 +101,31 @@ typedef struct {
     void *inv;
     int *st;
     int pcols;          /* columns of a panel-buffer tile: g * nb */
-    int split;          /* bf16 split update: planes after the FP32 panel */
+    int split;          /* SPLIT_*: operand terms after the FP32 panel */
     int blocked;        /* diagonal-block inverses only; blocked panel solve */
+    const float *rscale;    /* fp16 terms: row scales (in panel buffer 0) */
 } group_t;
 
 /* FP32 factorizations whose tiles are a multiple of 32 run their trailing
- * updates on bf16 MFMA: 1 with split operands (six products, when
- * enabled), 2 with bf16 inputs (one product, the low-precision factor of
- * plasma_dhposv), 0 on FP32 MFMA */
+ * updates on the 16-bit matrix cores: the low-precision input factor of the
+ * GMRES-IR solver takes one product of bf16 or fp16 inputs; otherwise
+ * PlasmaSplitUpdate chooses the split (PlasmaEnabled: the default, the
+ * three-term bf16 form) or FP32 MFMA (PlasmaDisabled). */
 static int use_split(plasma_desc_t A)
 {
     plasma_context_t *ctx = plasma_context_self();
     if (A.precision != PlasmaRealFloat || A.nb % 32 != 0 || A.mb != A.nb)
-        return 0;
-    if (ctx->bf16_factor)
-        return 2;
-    return ctx->split_update == PlasmaEnabled;
+        return SPLIT_NONE;
+    if (ctx->low_factor == PlasmaLowBf16)
+        return SPLIT_BF16X1;
+    if (ctx->low_factor == PlasmaLowFp16)
+        return SPLIT_FP16X1;
+    switch (ctx->split_update) {
+    case PlasmaEnabled:
+    case PlasmaSplitBf16:  return SPLIT_BF16X3;
+    case PlasmaSplitFp16:  return SPLIT_FP16X2;
+    default:               return SPLIT_NONE;
+    }
 }
 
 /* Bytes of the FP32 part of a panel buffer, rounded up so that the split
@@ -114,6 +135,18 @@ static size_t panel_fp_bytes(plasma_desc_t A, int pcols)
     const size_t b = (size_t)imax(A.mt - 1, 1) * A.mb * pcols *
                      plasma_element_size(A.precision);
     return (b + 255) / 256 * 256;
+}
+
+/* Bytes of the operand terms of a panel buffer. */
+static size_t planes_bytes(plasma_desc_t A, int pcols, int split)
+{
+    switch (split) {
+    case SPLIT_BF16X3:
+    case SPLIT_BF16X1: return plasma_split_bytes(A, pcols);
+    case SPLIT_FP16X2: return plasma_split16_bytes(A, pcols, 2);
+    case SPLIT_FP16X1: return plasma_split16_bytes(A, pcols, 1);
+    default:           return 0;
+    }
 }
 
 static void *planes_of(const group_t *g, const void *panel)
@@ -147,7 +180,12 @@ static int k_panel(const group_t *g, int k, void *panel, int pbase,
                                          (k - pbase) * g->A.nb, g->st, s)
         : plasma_kern_trsm_panel(g->uplo, g->A, k, g->inv, panel, pbase,
                                  g->pcols, (k - pbase) * g->A.nb, g->st, s);
-    if (!err && g->split)
+    if (!err && split_is_fp16(g->split))
+        err = plasma_kern_split16_panel(g->A, k, pbase, g->pcols, panel,
+                                        planes_of(g, panel),
+                                        g->split == SPLIT_FP16X2 ? 2 : 1,
+                                        g->rscale, g->st, s);
+    else if (!err && g->split)
         err = plasma_kern_split_panel(g->A, k, pbase, g->pcols, panel,
                                       planes_of(g, panel), g->st, s);
     plasma_prof_end(s, slot);
@@ -164,14 +202,20 @@ static int k_update(const group_t *g, int pbase, int ncols, int n0, int n1,
     plasma_prof_begin(s, PlasmaProfUpdate,
                       ctx->profile ? update_flops(g->A, kdepth, n0, n1) : 0,
                       0, &slot);
-    int err = g->split
-        ? plasma_kern_trailing_update_split(g->uplo, g->A, pbase, kdepth, n0,
-                                            n1, panel, planes_of(g, panel),
-                                            g->pcols, copy_cols, max_wgs,
-                                            g->split == 2, g->st, s)
-        : plasma_kern_trailing_update(g->uplo, g->A, pbase, kdepth, n0, n1,
-                                      panel, g->pcols, copy_cols, max_wgs,
-                                      g->st, s);
+    int err;
+    if (split_is_fp16(g->split))
+        err = plasma_kern_trailing_update_split16(
+            g->uplo, g->A, pbase, kdepth, n0, n1, panel, planes_of(g, panel),
+            g->pcols, copy_cols, max_wgs, g->split == SPLIT_FP16X2 ? 2 : 1,
+            g->rscale, g->st, s);
+    else if (g->split)
+        err = plasma_kern_trailing_update_split(
+            g->uplo, g->A, pbase, kdepth, n0, n1, panel, planes_of(g, panel),
+            g->pcols, copy_cols, max_wgs, g->split == SPLIT_BF16X1, g->st, s);
+    else
+        err = plasma_kern_trailing_update(g->uplo, g->A, pbase, kdepth, n0,
+                                          n1, panel, g->pcols, copy_cols,
+                                          max_wgs, g->st, s);
     plasma_prof_end(s, slot);
     return err;
 }
@@ -195,9 +239,39 @@ static int factor_group(const group_t *g, int c0, int w, void *panel,
     return 0;
 }
 
-/* The launch sequence proper (main and panel streams joined on return). */
-static int pxpotrf_groups(const group_t *gp, int lookahead, void *panel[2],
-                          hipEvent_t ev[3], plasma_sequence_t *sequence,
+/* Events of one factorization: the fork, a ring of "group factored" events
+ * (panel stream), a ring of "bulk update done" events (main stream), and the
+ * join.  Rings of depth + 2 slots: a wait always refers to the latest record
+ * of its slot (the slot is recorded again only depth + 2 groups later). */
+#define EV_RING (PLASMA_MAX_LOOKAHEAD + 2)
+#define PLASMA_FACTOR_EVENTS (2 * EV_RING + 2)
+
+/* The launch sequence proper (main and panel streams joined on return).
+ *
+ * depth 0: one stream; group q is factored (factor_group) and then updates
+ * the whole trailing matrix.
+ *
+ * depth d >= 1 (lookahead): the panel stream factors group t as soon as
+ * every update of its columns is done, and the main stream runs the bulk
+ * updates beside it:
+ *   panel stream, group t:  wait bulk(t - d - 1);
+ *                           narrow(i -> t) for i = t - d .. t - 1 (the
+ *                           updates of group t's columns by the d previous
+ *                           groups, left-looking, from their panel buffers);
+ *                           factor_group(t) into buffer t % (d + 1)
+ *   main stream, group t:   wait factor(t);
+ *                           bulk(t): columns of groups >= t + d + 1, and
+ *                           group t's panels copied back into A
+ * so the panel stream runs up to d groups ahead of the bulk updates
+ * (reference compute/pzpotrf.c:37-80: the DAG lets the runtime do the same
+ * with the potrf / trsm / syrk tasks of the next columns).  Races are
+ * excluded by construction: bulk(j) writes groups >= j + d + 1, so the
+ * bulk updates that touch group t are exactly those with j <= t - d - 1,
+ * all waited for; buffer t % (d + 1) was last read by group t - d - 1,
+ * whose narrow updates precede it on the panel stream and whose bulk
+ * update is waited for. */
+static int pxpotrf_groups(const group_t *gp, int depth, void *panel[],
+                          hipEvent_t ev[], plasma_sequence_t *sequence,
                           plasma_request_t *request)
 {
     plasma_context_t *ctx = plasma_context_self();
@@ -205,9 +279,10 @@ static int pxpotrf_groups(const group_t *gp, int lookahead, void *panel[2],
     const plasma_desc_t A = gp->A;
     const int mt = A.mt;
     const int gw = imax(1, imin(ctx->group, mt));
+    const int ngroups = (mt + gw - 1) / gw;
     const group_t g = *gp;
 
-    if (!lookahead) {
+    if (depth == 0) {
         for (int c0 = 0; c0 < mt; c0 += gw) {
             const int w = imin(gw, mt - c0);
             if (factor_group(&g, c0, w, panel[0], s0, sequence, request) ||
@@ -218,7 +293,8 @@ static int pxpotrf_groups(const group_t *gp, int lookahead, void *panel[2],
         return 0;
     }
 
-    hipEvent_t ev_fork = ev[0], ev_col = ev[1], ev_panel = ev[2];
+    hipEvent_t ev_fork = ev[0], ev_join = ev[1];
+    hipEvent_t *ev_factor = ev + 2, *ev_bulk = ev + 2 + EV_RING;
     /* The bulk update's grid: in FP32 (two update workgroups per CU) it is
      * capped to leave 64 workgroup slots to the lookahead stream, which
      * then starts its diagonal factorization and panel solve without
@@ -233,39 +309,34 @@ static int pxpotrf_groups(const group_t *gp, int lookahead, void *panel[2],
     hipEventRecord(ev_fork, s0);
     hipStreamWaitEvent(s1, ev_fork, 0);
 
-    int err = factor_group(&g, 0, imin(gw, mt), panel[0], s1, sequence,
-                           request);
-    hipEventRecord(ev_panel, s1);
-
-    for (int q = 0, c0 = 0; c0 < mt && !err; q++, c0 += gw) {
-        const int w = imin(gw, mt - c0);
-        const int c1 = c0 + w;               /* first column of group q+1 */
-        const int w1 = imin(gw, mt - c1);
-        void *pq = panel[q & 1];
-        /* main stream: wait for group q, update group q+1's columns */
-        hipStreamWaitEvent(s0, ev_panel, 0);
-        if (w1 > 0) {
-            err = issue(k_update(&g, c0, w, c1, c1 + w1, pq, 0, 0, s0),
+    int err = 0;
+    for (int t = 0; t < ngroups && !err; t++) {
+        const int c0 = t * gw, w = imin(gw, mt - c0);
+        void *pt = panel[t % (depth + 1)];
+        /* panel stream: the updates of group t's columns, then group t */
+        if (t - depth - 1 >= 0)
+            hipStreamWaitEvent(s1, ev_bulk[(t - depth - 1) % EV_RING], 0);
+        for (int i = imax(0, t - depth); i < t && !err; i++) {
+            const int ci = i * gw;
+            err = issue(k_update(&g, ci, imin(gw, mt - ci), c0, c0 + w,
+                                 panel[i % (depth + 1)], 0, 0, s1),
                         sequence, request);
-            hipEventRecord(ev_col, s0);
-            /* panel stream: factor group q+1 into the other buffer, whose
-             * last reader (group q-1's update and copy-back) precedes
-             * ev_col on the main stream */
-            hipStreamWaitEvent(s1, ev_col, 0);
-            if (!err)
-                err = factor_group(&g, c1, w1, panel[(q + 1) & 1], s1,
-                                   sequence, request);
-            hipEventRecord(ev_panel, s1);
         }
-        /* main stream: the rest of the trailing matrix, and group q's
-         * panels back into A */
         if (!err)
-            err = issue(k_update(&g, c0, w, c1 + w1, mt, pq, w, bulk_wgs,
-                                 s0), sequence, request);
+            err = factor_group(&g, c0, w, pt, s1, sequence, request);
+        hipEventRecord(ev_factor[t % EV_RING], s1);
+        /* main stream: group t's bulk update beyond its window, and its
+         * panels back into A */
+        hipStreamWaitEvent(s0, ev_factor[t % EV_RING], 0);
+        if (!err)
+            err = issue(k_update(&g, c0, w, imin(mt, (t + depth + 1) * gw),
+                                 mt, pt, w, bulk_wgs, s0),
+                        sequence, request);
+        hipEventRecord(ev_bulk[t % EV_RING], s0);
     }
     /* join */
-    hipEventRecord(ev_panel, s1);
-    hipStreamWaitEvent(s0, ev_panel, 0);
+    hipEventRecord(ev_join, s1);
+    hipStreamWaitEvent(s0, ev_join, 0);
     return err;
 }
 
@@ -274,8 +345,8 @@ static int pxpotrf_groups(const group_t *gp, int lookahead, void *panel[2],
  * when a launch failed (the sequence is marked).  ev: three events the
  * caller owns for the duration of the call (and of a capture). */
 static int pxpotrf_launch(plasma_enum_t uplo, plasma_desc_t A, void *inv,
-                          int want_w, void *panel[2], int lookahead, int *st,
-                          hipEvent_t ev[3], plasma_sequence_t *sequence,
+                          int want_w, void *panel[], int lookahead, int *st,
+                          hipEvent_t ev[], plasma_sequence_t *sequence,
                           plasma_request_t *request)
 {
     plasma_context_t *ctx = plasma_context_self();
@@ -289,8 +360,21 @@ static int pxpotrf_launch(plasma_enum_t uplo, plasma_desc_t A, void *inv,
      * full-inverse panel GEMM. */
     const int blocked = A.precision == PlasmaRealFloat && !lookahead &&
                         plasma_panel_blocked_ok(A);
-    const group_t g = {uplo, A, inv, st, gw * A.nb, use_split(A), blocked};
-    int err = pxpotrf_groups(&g, lookahead, panel, ev, sequence, request);
+    const int split = use_split(A);
+    const int pcols = gw * A.nb;
+    /* fp16 terms: the row scales follow the planes of panel buffer 0 */
+    float *rscale = split_is_fp16(split)
+        ? (float *)((char *)panel[0] + panel_fp_bytes(A, pcols) +
+                    planes_bytes(A, pcols, split))
+        : NULL;
+    const group_t g = {uplo, A, inv, st, pcols, split, blocked, rscale};
+    int err = 0;
+    if (rscale != NULL)
+        err = issue(plasma_kern_split16_scale(uplo, A, rscale, st,
+                                              ctx->stream),
+                    sequence, request);
+    if (!err)
+        err = pxpotrf_groups(&g, lookahead, panel, ev, sequence, request);
     /* the whole W_kk for a caller that keeps it (the blocked schedule only
      * formed its diagonal blocks): all tiles at once, off the critical
      * path */
@@ -305,45 +389,49 @@ static int pxpotrf_launch(plasma_enum_t uplo, plasma_desc_t A, void *inv,
 }
 
 /* Workspace of one factorization: the inverses (when the caller passes
- * none) and the panel buffers (two with lookahead).  Returns non-zero when
- * out of memory, with nothing left allocated. */
-static int pxpotrf_buffers(plasma_desc_t A, void *inv, int lookahead,
-                           void **own_inv, void *panel[2])
+ * none) and the panel buffers (depth + 1 with lookahead depth `depth`).
+ * Returns non-zero when out of memory, with nothing left allocated. */
+static int pxpotrf_buffers(plasma_desc_t A, void *inv, int depth,
+                           void **own_inv, void *panel[])
 {
     plasma_context_t *ctx = plasma_context_self();
     const int gw = imax(1, imin(ctx->group, A.mt));
+    const int split = use_split(A);
     const size_t pbytes =
-        panel_fp_bytes(A, gw * A.nb) +
-        (use_split(A) ? plasma_split_bytes(A, gw * A.nb) : 0);
+        panel_fp_bytes(A, gw * A.nb) + planes_bytes(A, gw * A.nb, split) +
+        (split_is_fp16(split) ? plasma_split16_scale_elems(A) * sizeof(float)
+                              : 0);
     *own_inv = NULL;
-    panel[0] = panel[1] = NULL;
+    for (int i = 0; i <= PLASMA_MAX_LOOKAHEAD; i++)
+        panel[i] = NULL;
     if (inv == NULL && (*own_inv = plasma_dev_alloc(plasma_inv_bytes(A))) ==
                            NULL)
         return 1;
-    panel[0] = plasma_dev_alloc(pbytes);
-    panel[1] = lookahead ? plasma_dev_alloc(pbytes) : panel[0];
-    if (panel[0] == NULL || panel[1] == NULL) {
-        if (panel[1] != panel[0])
-            plasma_dev_free(panel[1]);
-        plasma_dev_free(panel[0]);
-        plasma_dev_free(*own_inv);
-        *own_inv = NULL;
-        return 1;
+    for (int i = 0; i <= depth; i++) {
+        panel[i] = plasma_dev_alloc(pbytes);
+        if (panel[i] == NULL) {
+            while (i-- > 0) {
+                plasma_dev_free(panel[i]);
+                panel[i] = NULL;
+            }
+            plasma_dev_free(*own_inv);
+            *own_inv = NULL;
+            return 1;
+        }
     }
     return 0;
 }
 
-static void pxpotrf_free(void *own_inv, void *panel[2])
+static void pxpotrf_free(void *own_inv, void *panel[])
 {
-    if (panel[1] != panel[0])
-        plasma_dev_free(panel[1]);
-    plasma_dev_free(panel[0]);
+    for (int i = 0; i <= PLASMA_MAX_LOOKAHEAD; i++)
+        plasma_dev_free(panel[i]);
     plasma_dev_free(own_inv);
 }
 
-static int events_create(hipEvent_t ev[3])
+static int events_create(hipEvent_t ev[PLASMA_FACTOR_EVENTS])
 {
-    for (int i = 0; i < 3; i++)
+    for (int i = 0; i < PLASMA_FACTOR_EVENTS; i++)
         if (plasma_hip_check(hipEventCreateWithFlags(&ev[i],
                                                      hipEventDisableTiming),
                              "hipEventCreate")) {
@@ -354,9 +442,9 @@ static int events_create(hipEvent_t ev[3])
     return 0;
 }
 
-static void events_destroy(hipEvent_t ev[3])
+static void events_destroy(hipEvent_t ev[PLASMA_FACTOR_EVENTS])
 {
-    for (int i = 0; i < 3; i++)
+    for (int i = 0; i < PLASMA_FACTOR_EVENTS; i++)
         hipEventDestroy(ev[i]);
 }
 
@@ -412,12 +500,12 @@ static plasma_graph_entry_t *graph_get(plasma_enum_t uplo, plasma_desc_t A,
         graph_drop(victim);
     }
     e = victim;
-    void *own_inv, *panel[2];
+    void *own_inv, *panel[PLASMA_MAX_LOOKAHEAD + 1];
     if (pxpotrf_buffers(A, inv, lookahead, &own_inv, panel)) {
         plasma_request_fail(sequence, request, PlasmaErrorOutOfMemory);
         return NULL;
     }
-    hipEvent_t ev[3];
+    hipEvent_t ev[PLASMA_FACTOR_EVENTS];
     if (events_create(ev)) {
         pxpotrf_free(own_inv, panel);
         plasma_request_fail(sequence, request, PlasmaErrorInternal);
@@ -456,8 +544,8 @@ static plasma_graph_entry_t *graph_get(plasma_enum_t uplo, plasma_desc_t A,
     e->group = ctx->group;
     e->split = use_split(A);
     e->own_inv = own_inv;
-    e->panel[0] = panel[0];
-    e->panel[1] = panel[1];
+    for (int i = 0; i <= PLASMA_MAX_LOOKAHEAD; i++)
+        e->panel[i] = panel[i];
     return e;
 }
 
@@ -470,7 +558,8 @@ void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A, void *inv,
     int *st = plasma_sequence_dstatus(sequence, request);
     hipStream_t s0 = ctx->stream;
     const int gw = imax(1, imin(ctx->group, A.mt));
-    const int lookahead = ctx->lookahead > 0 && A.mt > gw;
+    /* lookahead depth: at most the number of groups after the first */
+    const int lookahead = imin(ctx->lookahead, (A.mt + gw - 1) / gw - 1);
 
     if (ctx->graph_capture == PlasmaEnabled && !ctx->profile) {
         plasma_graph_entry_t *e = graph_get(uplo, A, inv, lookahead,
@@ -486,12 +575,12 @@ void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A, void *inv,
         return;
     }
 
-    void *own_inv, *panel[2];
+    void *own_inv, *panel[PLASMA_MAX_LOOKAHEAD + 1];
     if (pxpotrf_buffers(A, inv, lookahead, &own_inv, panel)) {
         plasma_request_fail(sequence, request, PlasmaErrorOutOfMemory);
         return;
     }
-    hipEvent_t ev[3];
+    hipEvent_t ev[PLASMA_FACTOR_EVENTS];
     if (events_create(ev)) {
         pxpotrf_free(own_inv, panel);
         plasma_request_fail(sequence, request, PlasmaErrorInternal);

@@ -59,7 +59,8 @@ static void context_defaults(plasma_context_t *ctx)
     ctx->lookahead = 1;
     ctx->group = 2;
     ctx->split_update = PlasmaEnabled;
-    ctx->bf16_factor = 0;
+    ctx->half_input = PlasmaSplitBf16;
+    ctx->low_factor = 0;
     ctx->graph_capture = PlasmaDisabled;
 }
 
@@ -206,11 +207,19 @@ int plasma_set(plasma_enum_t param, int value)
         ctx->graph_capture = value;
         break;
     case PlasmaSplitUpdate:
-        if (value != PlasmaEnabled && value != PlasmaDisabled) {
-            plasma_error("invalid split update flag");
+        if (value != PlasmaEnabled && value != PlasmaDisabled &&
+            value != PlasmaSplitBf16 && value != PlasmaSplitFp16) {
+            plasma_error("invalid split update mode");
             return PlasmaErrorIllegalValue;
         }
         ctx->split_update = value;
+        break;
+    case PlasmaHalfInput:
+        if (value != PlasmaSplitBf16 && value != PlasmaSplitFp16) {
+            plasma_error("invalid half-precision input format");
+            return PlasmaErrorIllegalValue;
+        }
+        ctx->half_input = value;
         break;
     default:
         plasma_error("unknown parameter");
@@ -235,6 +244,7 @@ int plasma_get(plasma_enum_t param, int *value)
     case PlasmaDevice:          *value = ctx->device;            break;
     case PlasmaUpdateDepth:     *value = ctx->group;             break;
     case PlasmaSplitUpdate:     *value = ctx->split_update;      break;
+    case PlasmaHalfInput:       *value = ctx->half_input;        break;
     default:
         plasma_error("Unknown parameter");
         return PlasmaErrorIllegalValue;

@@ -42,6 +42,10 @@ typedef struct {
     int mode;                 /* 0 any, 1 eager launches, 2 graph replay */
 } plasma_tune_row_t;
 
+/* Deepest panel lookahead (PlasmaLookahead): the panel stream runs up to
+ * this many column groups ahead of the bulk trailing update. */
+#define PLASMA_MAX_LOOKAHEAD 4
+
 /* Cached factorization graphs (compute/pxpotrf.c). */
 #define PLASMA_GRAPHS 4
 typedef struct {
@@ -53,7 +57,7 @@ typedef struct {
     int lookahead, group, split;
     hipGraphExec_t exec;
     void *own_inv;            /* buffers the graph's launches address */
-    void *panel[2];
+    void *panel[PLASMA_MAX_LOOKAHEAD + 1];
 } plasma_graph_entry_t;
 
 /* profiler records (control/profile.c) */
@@ -77,10 +81,14 @@ struct plasma_context_s {
     /* extensions */
     int lookahead;        /* panel lookahead depth, 0 disables */
     int group;            /* tile columns whose panels update together */
-    int split_update;     /* FP32 trailing updates on bf16 MFMA: panels
-                             split into three bf16 terms, six products */
-    int bf16_factor;      /* set by plasma_dhposv around its factorization:
-                             FP32 trailing updates from bf16 inputs */
+    int split_update;     /* FP32 trailing updates on 16-bit MFMA:
+                             PlasmaSplitFp16 / PlasmaSplitBf16 /
+                             PlasmaEnabled (default) / PlasmaDisabled */
+    int half_input;       /* plasma_dhposv's factor input format:
+                             PlasmaSplitFp16 or PlasmaSplitBf16 */
+    int low_factor;       /* set by plasma_dhposv around its factorization:
+                             FP32 trailing updates from one-term 16-bit
+                             inputs (PlasmaLowBf16 / PlasmaLowFp16), 0 off */
     int graph_capture;    /* replay cached hipGraphs for the sync drivers */
 
     /* device */
@@ -90,9 +98,10 @@ struct plasma_context_s {
     hipStream_t panel_stream;  /* lookahead stream: diagonal potrf + panel */
     int owns_streams;
 
-    /* status words, one per sequence */
+    /* status words, one per sequence with work in flight; a slot is free
+     * when slot_seq is NULL (control/async.c) */
     int *d_status;                                   /* device array */
-    const plasma_sequence_t *slot_seq[PLASMA_STATUS_SLOTS];
+    plasma_sequence_t *slot_seq[PLASMA_STATUS_SLOTS];
     plasma_request_t *slot_req[PLASMA_STATUS_SLOTS];
     int slot_next;
 
@@ -123,6 +132,10 @@ struct plasma_context_s {
 };
 
 #define PLASMA_SCALARS 64
+
+/* values of plasma_context_t.low_factor */
+#define PlasmaLowBf16 1
+#define PlasmaLowFp16 2
 
 
 void plasma_prof_begin(hipStream_t s, int cls, double flops, double bytes,

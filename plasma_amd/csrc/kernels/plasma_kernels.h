@@ -232,6 +232,29 @@ int plasma_kern_trailing_update_split(plasma_enum_t uplo, plasma_desc_t A,
                                       int copy_cols, int max_wgs, int single,
                                       const int *status, hipStream_t s);
 
+/* FP32 trailing updates on fp16 MFMA (kernels/split16.hip).  Rows are
+ * scaled by sigma_r = 2^-ceil(log2 sqrt(a_rr)) (plasma_kern_split16_scale,
+ * plasma_split16_scale_elems(A) floats, from the diagonal of A before the
+ * factorization), so every scaled factor entry has magnitude <= 1; each
+ * panel element is then stored as np fp16 terms (np = 2: hi + lo, three
+ * products per update product; np = 1: one product) in MFMA-fragment
+ * order (plasma_split16_bytes(A, pcols, np) bytes per panel buffer).  The
+ * update scales C by sigma_r sigma_c on entry and back on exit. */
+size_t plasma_split16_bytes(plasma_desc_t A, int pcols, int np);
+size_t plasma_split16_scale_elems(plasma_desc_t A);
+int plasma_kern_split16_scale(plasma_enum_t uplo, plasma_desc_t A,
+                              float *rscale, const int *status,
+                              hipStream_t s);
+int plasma_kern_split16_panel(plasma_desc_t A, int k, int pbase, int pcols,
+                              const void *panel, void *planes, int np,
+                              const float *rscale, const int *status,
+                              hipStream_t s);
+int plasma_kern_trailing_update_split16(
+    plasma_enum_t uplo, plasma_desc_t A, int pbase, int kdepth, int n_first,
+    int n_end, const void *panel, const void *planes, int pcols,
+    int copy_cols, int max_wgs, int np, const float *rscale,
+    const int *status, hipStream_t s);
+
 /*
  * Mixed-precision helpers (reference compute/zcposv.c:363-539).
  */
@@ -250,6 +273,18 @@ int plasma_kern_geadd(double alpha, plasma_desc_t A, double beta,
  * a reduction); work: plasma_lansy_work_doubles(A) doubles of device memory;
  * result -> *value (device). */
 size_t plasma_lansy_work_doubles(plasma_desc_t A);
+/* Norm of a tile matrix (kernels/norms.hip): uplo PlasmaGeneral for any
+ * matrix, PlasmaLower / PlasmaUpper for a symmetric one held in that
+ * triangle; norm Max / One / Inf / Frobenius; *value (device) written in
+ * stream order; work: plasma_norm_work_doubles(A) doubles of device
+ * memory.  amax: per column (n values) or per row (mt * mb slots, the
+ * first m meaningful) max |a_ij| into device values. */
+size_t plasma_norm_work_doubles(plasma_desc_t A);
+int plasma_kern_norm(plasma_enum_t norm, plasma_enum_t uplo, plasma_desc_t A,
+                     double *work, double *value, const int *status,
+                     hipStream_t s);
+int plasma_kern_amax(plasma_enum_t colrow, plasma_desc_t A, double *values,
+                     const int *status, hipStream_t s);
 int plasma_kern_lansy_inf(plasma_enum_t uplo, plasma_desc_t A, double *work,
                           double *value, const int *status, hipStream_t s);
 /* values[j] = max_i |A(i,j)| (reference pdzamax.c:36-52) */

@@ -12,9 +12,14 @@ _mkl = None
 
 
 def mkl():
-    """MKL runtime via dlopen (None when absent)."""
+    """MKL runtime via dlopen (None when absent).  Its sequential threading
+    layer is selected before any MKL routine runs: MKL keeps the layer of
+    its first call, and the oracle's MKL backend needs the sequential one
+    (the Intel-OpenMP layer computes wrong results next to torch's
+    libgomp)."""
     global _mkl
     if _mkl is None:
+        os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
         for p in ("/opt/conda/lib/libmkl_rt.so.1", "libmkl_rt.so.1"):
             try:
                 _mkl = ctypes.CDLL(p)
@@ -23,6 +28,9 @@ def mkl():
                 continue
         else:
             _mkl = False
+        if _mkl:
+            _mkl.MKL_Set_Threading_Layer.restype = ctypes.c_int
+            _mkl.MKL_Set_Threading_Layer(1)   # MKL_THREADING_SEQUENTIAL
     return _mkl or None
 
 
@@ -61,6 +69,32 @@ def gp_kernel(n, d=8, lengthscale=1.0, noise=0.1, seed=0):
     D2 = np.maximum(sq[:, None] + sq[None, :] - 2.0 * (X @ X.T), 0.0)
     K = np.exp(-D2 / (2.0 * lengthscale ** 2)) + noise * np.eye(n)
     return np.asfortranarray(K)
+
+
+def householder_spd(n, kappa, nrefl=64, seed=0):
+    """A = Q diag(s) Q^T with Q = H_1 ... H_nrefl (Householder reflectors of
+    N(0,1) vectors) and s_i = kappa^(-i/(n-1)), so ||A||_2 = 1 and
+    cond_2(A) = kappa: the mixed-precision benchmark's matrices, formed in
+    O(n^2 nrefl) through the compact WY form Q = I - V T V^T."""
+    rng = np.random.default_rng(seed)
+    V = rng.standard_normal((n, nrefl))
+    s = kappa ** (-np.arange(n) / max(n - 1.0, 1.0))
+    G = V.T @ V
+    tau = 2.0 / np.diag(G)
+    T = np.zeros((nrefl, nrefl))
+    for j in range(nrefl):
+        T[j, j] = tau[j]
+        if j:
+            T[:j, j] = -tau[j] * (T[:j, :j] @ G[:j, j])
+    # A = D - U W^T - W U^T + U (V^T D V) U^T with U = V T, W = D V
+    Uv = V @ T
+    W = V * s[:, None]
+    M = V.T @ W
+    A = Uv @ (M @ Uv.T - W.T)
+    A -= W @ Uv.T
+    A[np.diag_indices(n)] += s
+    A = (A + A.T) * 0.5
+    return np.asfortranarray(A)
 
 
 def rhs(n, nrhs, seed=1, dtype=np.float64):

@@ -8,6 +8,10 @@ import sys
 
 import pytest
 
+# The oracle's MKL tile kernels must run on MKL's sequential threading layer,
+# which MKL fixes at its first routine call in the process.
+os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
@@ -35,7 +39,7 @@ def nb_guard(plasma):
     import plasma_amd as pa
     params = (pa.PlasmaNb, pa.PlasmaTuning, pa.PlasmaLookahead,
               pa.PlasmaUpdateDepth, pa.PlasmaGraphCapture,
-              pa.PlasmaSplitUpdate)
+              pa.PlasmaSplitUpdate, pa.PlasmaHalfInput)
     saved = [pa.get_param(p)[1] for p in params]
     # a test that sets nb means that nb: the autotuned table (on by default,
     # as in the reference) is switched off unless the test enables it

@@ -119,27 +119,67 @@ def test_dpotrf_not_pd(plasma, nb_guard, uplo, zerocol):
 
 @pytest.mark.parametrize("uplo", [L, Up])
 @pytest.mark.parametrize("n,nb,depth", [(1000, 128, 2), (2048, 256, 1),
-                                        (1500, 96, 3), (700, 512, 1)])
-def test_spotrf_split_update(plasma, nb_guard, uplo, n, nb, depth):
-    """FP32 trailing updates on bf16 MFMA with operands split into three
-    bf16 terms: the factor matches the FP32-MFMA one to FP32 accuracy
-    (error vs the oracle within 3x of the plain FP32 path's, and <= 1e-5)."""
+                                        (1500, 96, 3), (700, 512, 1),
+                                        (1300, 256, 4)])
+@pytest.mark.parametrize("mode", ["bf16", "fp16"])
+def test_spotrf_split_update(plasma, nb_guard, uplo, n, nb, depth, mode):
+    """FP32 trailing updates on 16-bit MFMA from exact operand splits
+    (three bf16 terms and six products, or row-scaled two-term fp16 and
+    three products): the factor matches the FP32-MFMA one to FP32 accuracy
+    (error vs the oracle within 4x of the plain FP32 path's, and <= 1e-5)."""
     pa.set_param(pa.PlasmaNb, nb)
     pa.set_param(pa.PlasmaUpdateDepth, depth)
     A = U.ggt_spd(n, seed=n, dtype=np.float32)
     ref, _ = ora.potrf(ora.LOWER if uplo == L else ora.UPPER, A, nb,
                        dtype=np.float32)
     errs = []
-    for flag in (pa.PlasmaDisabled, pa.PlasmaEnabled):
+    split = pa.PlasmaSplitBf16 if mode == "bf16" else pa.PlasmaSplitFp16
+    for flag in (pa.PlasmaDisabled, split):
         pa.set_param(pa.PlasmaSplitUpdate, flag)
         a = A.copy(order="F")
         assert pa.potrf(uplo, n, a, n, prec="s") == 0
         errs.append(U.factor_error(a, ref, uplo == L))
-    assert errs[1] <= 1e-5 and errs[1] <= 3 * errs[0] + 1e-7, errs
+    assert errs[1] <= 1e-5 and errs[1] <= 4 * errs[0] + 1e-7, errs
 
 
 @pytest.mark.parametrize("uplo", [L, Up])
 @pytest.mark.parametrize("lookahead", [0, 1])
+def test_spotrf_fp16_split_badly_scaled(plasma, nb_guard, uplo, lookahead):
+    """Rows scaled over twelve orders of magnitude (A = D G D, D log-spaced
+    in [1e-6, 1e6]): the fp16 terms are formed after scaling each row of
+    the factor by a power of two that bounds it by 1, so the factor keeps
+    FP32 accuracy row by row.  Checked against the oracle's FP32 factor
+    row-wise relative to each row's scale, next to the FP32-MFMA path."""
+    n, nb = 1024, 256
+    pa.set_param(pa.PlasmaNb, nb)
+    pa.set_param(pa.PlasmaLookahead, lookahead)
+    d = np.logspace(-6, 6, n)
+    rng = np.random.default_rng(5)
+    perm = rng.permutation(n)
+    d = d[perm]
+    A64 = U.ggt_spd(n, seed=3) * d[:, None] * d[None, :]
+    A = np.asfortranarray(A64.astype(np.float32))
+    ref, _ = ora.potrf(ora.LOWER if uplo == L else ora.UPPER, A, nb,
+                       dtype=np.float32)
+    low = uplo == L
+    tri = np.tril if low else np.triu
+    dscale = d if low else d            # row r of L (col r of U) ~ d_r
+    errs = []
+    for flag in (pa.PlasmaDisabled, pa.PlasmaSplitFp16):
+        pa.set_param(pa.PlasmaSplitUpdate, flag)
+        a = A.copy(order="F")
+        assert pa.potrf(uplo, n, a, n, prec="s") == 0
+        diff = tri(a.astype(np.float64) - ref.astype(np.float64))
+        if not low:
+            diff = diff.T
+        # row r of L scales like d_r: relative error per row
+        rel = np.abs(diff).max(axis=1) / dscale
+        errs.append(rel.max())
+    assert errs[1] <= 1e-4 and errs[1] <= 8 * errs[0] + 1e-7, errs
+
+
+@pytest.mark.parametrize("uplo", [L, Up])
+@pytest.mark.parametrize("lookahead", [0, 1, 3])
 @pytest.mark.parametrize("n,nb", [(1000, 128), (1300, 256)])
 def test_spotrf_panel_schedules(plasma, nb_guard, uplo, lookahead, n, nb):
     """FP32 without lookahead solves its panels by 32-column block
@@ -232,7 +272,7 @@ def test_spotrf_sposv(plasma, nb_guard, uplo, n, nb):
     assert res < 10 * U.EPS32
 
 
-@pytest.mark.parametrize("lookahead", [0, 1])
+@pytest.mark.parametrize("lookahead", [0, 1, 2, 4])
 def test_dposv_8192(plasma, nb_guard, lookahead):
     """A full-size-class solve on device memory: residual and the factor
     against the oracle (MKL tiles on the host)."""
@@ -256,14 +296,15 @@ def test_dposv_8192(plasma, nb_guard, lookahead):
 
 @pytest.mark.parametrize("uplo", [L, Up])
 @pytest.mark.parametrize("depth", [1, 2, 3, 4])
-@pytest.mark.parametrize("lookahead", [0, 1])
+@pytest.mark.parametrize("lookahead", [0, 1, 2, 3])
 @pytest.mark.parametrize("n,nb", [(1000, 128), (1152, 128), (300, 64)])
 def test_dpotrf_update_depth(plasma, nb_guard, uplo, depth, lookahead, n,
                              nb):
-    """Trailing updates by groups of 1-4 panels (K = depth * nb), with and
-    without lookahead, including a short last tile row (n % nb != 0) and
-    tile counts that are not a multiple of the group size: the factor
-    against the oracle and the untouched triangle."""
+    """Trailing updates by groups of 1-4 panels (K = depth * nb), without
+    lookahead and with the panel stream 1-3 groups ahead, including a short
+    last tile row (n % nb != 0), tile counts that are not a multiple of the
+    group size and more lookahead than groups: the factor against the
+    oracle and the untouched triangle."""
     pa.set_param(pa.PlasmaTuning, pa.PlasmaDisabled)
     pa.set_param(pa.PlasmaNb, nb)
     pa.set_param(pa.PlasmaLookahead, lookahead)

@@ -28,7 +28,7 @@ def _device(a):
 
 
 @pytest.mark.parametrize("uplo", [L, Up])
-@pytest.mark.parametrize("lookahead", [0, 1])
+@pytest.mark.parametrize("lookahead", [0, 1, 3])
 def test_graph_replays(graph_mode, uplo, lookahead):
     import torch
     n, nb = 1000, 128

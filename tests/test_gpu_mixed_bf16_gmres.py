@@ -1,6 +1,7 @@
 """GPU tests of plasma_dhposv (no reference counterpart): the FP64 solution
-of A X = B from a bf16-input factor of the equilibrated matrix, refined by
-GMRES-IR, through the C ABI.
+of A X = B from a half-precision-input factor (fp16 by default, bf16 with
+PlasmaHalfInput) of the equilibrated matrix, refined by GMRES-IR, through
+the C ABI.
 
 Pinned by what fixes the answer independently: the tester's scaled
 residual ||B - A X||_inf / (n ||A||_inf ||X||_inf eps64) < 10
@@ -44,8 +45,11 @@ def _dhposv(uplo, A, B):
 @pytest.mark.parametrize("uplo", [L, Up])
 @pytest.mark.parametrize("kappa", [1e2, 1e4, 1e6])
 @pytest.mark.parametrize("n,nb", [(512, 128), (700, 128)])
-def test_dhposv_accuracy(plasma, nb_guard, uplo, kappa, n, nb):
+@pytest.mark.parametrize("half", ["fp16", "bf16"])
+def test_dhposv_accuracy(plasma, nb_guard, uplo, kappa, n, nb, half):
     pa.set_param(pa.PlasmaNb, nb)
+    pa.set_param(pa.PlasmaHalfInput, pa.PlasmaSplitFp16 if half == "fp16"
+                 else pa.PlasmaSplitBf16)
     A = _scaled_spd(n, kappa, seed=3)
     B = U.rhs(n, 6, seed=4)
     X, it, info = _dhposv(uplo, A, B)
@@ -63,12 +67,15 @@ def test_dhposv_accuracy(plasma, nb_guard, uplo, kappa, n, nb):
     assert err < max(1e-10, kappa * 1e8 * 1e-15), err
 
 
-def test_dhposv_breakdown_falls_back(plasma, nb_guard):
-    """kappa = 1e10: the bf16-input factor meets a pivot that is not
-    positive (or refinement stalls); the FP64 factorization solves and iter
-    is -1."""
+@pytest.mark.parametrize("half", ["fp16", "bf16"])
+def test_dhposv_breakdown_falls_back(plasma, nb_guard, half):
+    """kappa = 1e10: the half-precision-input factor meets a pivot that is
+    not positive (or refinement stalls); the FP64 factorization solves and
+    iter is -1."""
     n = 384
     pa.set_param(pa.PlasmaNb, 128)
+    pa.set_param(pa.PlasmaHalfInput, pa.PlasmaSplitFp16 if half == "fp16"
+                 else pa.PlasmaSplitBf16)
     A = _cond_spd(n, 1e10, seed=5)
     B = U.rhs(n, 3, seed=6)
     X, it, info = _dhposv(L, A, B)

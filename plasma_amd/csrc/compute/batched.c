@@ -102,45 +102,38 @@ static int vbatched(plasma_enum_t precision, plasma_enum_t uplo, int batch,
         return -9;
     }
 
-    /* host views of n and aoff: the order and the workspace size */
+    /* host view of n: the argument check and the order */
     const size_t nbytes = (size_t)batch * sizeof(int);
     const size_t obytes = (size_t)batch * sizeof(size_t);
     int *hn = malloc(nbytes);
-    size_t *hoff = malloc(obytes);
     order_t *ord = malloc((size_t)batch * sizeof(order_t));
     int *hord = malloc(nbytes);
     int retval = PlasmaSuccess;
     void *dn = NULL, *doff = NULL, *dboff = NULL, *dinfo = NULL;
     int own_n = 0, own_off = 0, own_boff = 0, own_info = 0;
     void *W = NULL, *dord = NULL, *dticket = NULL;
-    if (hn == NULL || hoff == NULL || ord == NULL || hord == NULL) {
+    if (hn == NULL || ord == NULL || hord == NULL) {
         retval = PlasmaErrorOutOfMemory;
         goto done;
     }
-    /* a caller's device arrays may have been written by work queued on the
-     * library's streams (non-blocking, so the copies below would not wait
-     * for it) */
-    if (plasma_is_device_ptr(n) || plasma_is_device_ptr(aoff)) {
+    /* a caller's device array may have been written by work queued on the
+     * library's streams (non-blocking, so the copy below would not wait for
+     * it) */
+    if (plasma_is_device_ptr(n)) {
         hipStreamSynchronize(ctx->panel_stream);
         hipStreamSynchronize(ctx->stream);
     }
     if (plasma_hip_check(hipMemcpy(hn, n, nbytes, hipMemcpyDefault),
-                         "hipMemcpy(n)") ||
-        plasma_hip_check(hipMemcpy(hoff, aoff, obytes, hipMemcpyDefault),
-                         "hipMemcpy(aoff)")) {
+                         "hipMemcpy(n)")) {
         retval = PlasmaErrorInternal;
         goto done;
     }
-    size_t wsize = 0;
     for (int i = 0; i < batch; i++) {
         if (hn[i] < 0 || hn[i] > VBATCH_NMAX) {
             plasma_error("illegal order of a system");
             retval = -3;
             goto done;
         }
-        size_t end = hoff[i] + (size_t)hn[i] * hn[i];
-        if (end > wsize)
-            wsize = end;
         ord[i].n = hn[i];
         ord[i].i = i;
     }
@@ -148,8 +141,16 @@ static int vbatched(plasma_enum_t precision, plasma_enum_t uplo, int batch,
     for (int i = 0; i < batch; i++)
         hord[i] = ord[i].i;
 
-    const size_t es = plasma_element_size(precision);
-    W = plasma_dev_alloc(wsize * es);
+    /* one resident grid: as many workgroups as the LDS lets the CUs hold,
+     * each with its own slab of W */
+    const size_t lds = plasma_kern_vbatched_lds(precision);
+    int per_cu = (int)((160 * 1024) / lds);
+    if (per_cu < 1)
+        per_cu = 1;
+    int grid = ctx->num_cus * per_cu;
+    if (grid > batch)
+        grid = batch;
+    W = plasma_dev_alloc((size_t)grid * plasma_kern_vbatched_wslab(precision));
     dord = plasma_dev_alloc(nbytes);
     dticket = plasma_dev_alloc(sizeof(int));
     if (W == NULL || dord == NULL || dticket == NULL) {
@@ -181,14 +182,6 @@ static int vbatched(plasma_enum_t precision, plasma_enum_t uplo, int batch,
         retval = PlasmaErrorInternal;
         goto done;
     }
-    /* one resident grid: as many workgroups as the LDS lets the CUs hold */
-    const size_t lds = plasma_kern_vbatched_lds(precision);
-    int per_cu = (int)((160 * 1024) / lds);
-    if (per_cu < 1)
-        per_cu = 1;
-    int grid = ctx->num_cus * per_cu;
-    if (grid > batch)
-        grid = batch;
     if (plasma_kern_vbatched_posv(precision, uplo == PlasmaUpper, batch, dn,
                                   A, doff, W, nrhs, B, dboff, dinfo, dord,
                                   dticket, grid, s) != 0) {
@@ -218,7 +211,6 @@ done:
     plasma_dev_free(W);
     free(hord);
     free(ord);
-    free(hoff);
     free(hn);
     return retval;
 }

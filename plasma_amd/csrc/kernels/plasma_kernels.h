@@ -137,13 +137,16 @@ int plasma_kern_diag_normal(plasma_desc_t A, plasma_desc_t B, const void *inv,
  * Batched SPD solves of uneven order (no reference counterpart).  System i
  * of order n[i] <= 512 is A + aoff[i] (column major, ld n[i], the uplo
  * triangle), its right-hand sides B + boff[i] (n[i] x nrhs, ld n[i]); W
- * has the layout of A (workspace for the diagonal-block inverses).  A
+ * holds one slab of plasma_kern_vbatched_wslab(precision) bytes per
+ * workgroup (the diagonal-block inverses of the system it is solving).  A
  * persistent grid of `grid` workgroups takes systems from *ticket (zero on
- * entry) in the order `order`; info[i] gets 0 or LAPACK's info.  All
- * pointers are device memory.  plasma_kern_vbatched_lds: LDS bytes per
- * workgroup (dynamic and static), for sizing the grid.
+ * entry) in the order `order` (NULL: index order); info[i] gets 0,
+ * LAPACK's info, or -3 for an order outside [0, 512].  All pointers are
+ * device memory.  plasma_kern_vbatched_lds: LDS bytes per workgroup
+ * (dynamic and static), for sizing the grid.
  */
 size_t plasma_kern_vbatched_lds(plasma_enum_t precision);
+size_t plasma_kern_vbatched_wslab(plasma_enum_t precision);
 int plasma_kern_vbatched_posv(plasma_enum_t precision, int upper, int batch,
                               const int *n, void *A, const size_t *aoff,
                               void *W, int nrhs, void *B, const size_t *boff,

@@ -898,9 +898,19 @@ int launch_trailing_update(plasma_enum_t uplo, plasma_desc_t A, int pbase,
 // diagonal blocks), then the two triangular solves by 32-row blocks,
 // y_J = W_JJ (b_J - L_J,<J y_<J) and x_J = W_JJ^T (y_J - L_>J,J^T x_>J),
 // with the solution vector in LDS.  A persistent grid takes systems from a
-// ticket counter in the order `order` (largest first); every workgroup
-// exits when the tickets run out.
+// ticket counter in the order `order` (largest first; NULL: index order);
+// every workgroup exits when the tickets run out.
+//
+// Workspace: only the 32 x 32 diagonal blocks of W are formed (mode 2), so
+// each workgroup keeps them in its own slab of W, reused system after
+// system: the view (r, c) -> r + 32 c puts block J, (r, c) in
+// [32 J, 32 J + 32)^2, at 1056 J + (r - 32 J) + 32 (c - 32 J), inside
+// [1056 J, 1056 J + 1024), so the blocks of an order-512 system fit in
+// VB_WSLAB = 33 * 512 elements.  A system whose order is outside [0, 512]
+// is skipped with info -3 (the argument position of n).
 // ---------------------------------------------------------------------------
+constexpr int VB_NMAX = 512;
+constexpr int VB_WSLAB = 33 * VB_NMAX;
 template <typename T>
 __device__ void solve_tile(const View<T> &L, const int n, const View<T> &Wv,
                            T *Bp, const int ldb, const int nrhs, T *sw)
@@ -980,18 +990,18 @@ __global__ __launch_bounds__(NTHREADS) void vbatched_kernel(
         const int i = s_i;
         __syncthreads();
         if (i >= batch) break;
-        const int sys = order[i];
+        const int sys = order != nullptr ? order[i] : i;
         const int n = nvec[sys];
-        int inf = 0;
-        if (n > 0) {
+        int inf = (n < 0 || n > VB_NMAX) ? -3 : 0;
+        if (n > 0 && inf == 0) {
             View<T> L;
             L.p = A + aoff[sys];
             L.rs = upper ? n : 1;
             L.cs = upper ? 1 : n;
             View<T> Wv;
-            Wv.p = W + aoff[sys];
+            Wv.p = W + (size_t)blockIdx.x * VB_WSLAB;
             Wv.rs = 1;
-            Wv.cs = n;
+            Wv.cs = JB;
             inf = factor_tile<T>(L, n, Wv, 2, sP);
             if (inf == 0 && nrhs > 0)
                 solve_tile<T>(L, n, Wv, B + boff[sys], n, nrhs, sP);
@@ -1125,6 +1135,11 @@ extern "C" size_t plasma_kern_vbatched_lds(plasma_enum_t precision)
     const size_t es = precision == PlasmaRealDouble ? 8 : 4;
     return (size_t)8 * JB * 2 * (JB + 1) * es +
            (2 * JB * (JB + 1) + 2 * JB) * es + 32;
+}
+
+extern "C" size_t plasma_kern_vbatched_wslab(plasma_enum_t precision)
+{
+    return (size_t)VB_WSLAB * (precision == PlasmaRealDouble ? 8 : 4);
 }
 
 extern "C" int plasma_kern_vbatched_posv(plasma_enum_t precision, int upper,

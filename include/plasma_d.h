@@ -62,6 +62,18 @@ void plasma_omp_dtr2desc(double *pA, int lda, plasma_desc_t A,
 void plasma_omp_ddesc2tr(plasma_desc_t A, double *pA, int lda,
                          plasma_sequence_t *sequence, plasma_request_t *request);
 
+/* Inverse of an SPD matrix: from its Cholesky factor (reference
+ * compute/zpotri.c:65, :206) or from the matrix itself (compute/zpoinv.c:67,
+ * :201); pA / A end with A^{-1} in the uplo triangle.  Return codes as
+ * plasma_dpotrf (-1 uplo, -2 n, -4 lda; i > 0: a zero at position i of the
+ * factor's diagonal, or potrf's info). */
+int plasma_dpotri(plasma_enum_t uplo, int n, double *pA, int lda);
+int plasma_dpoinv(plasma_enum_t uplo, int n, double *pA, int lda);
+void plasma_omp_dpotri(plasma_enum_t uplo, plasma_desc_t A,
+                       plasma_sequence_t *sequence, plasma_request_t *request);
+void plasma_omp_dpoinv(plasma_enum_t uplo, plasma_desc_t A,
+                       plasma_sequence_t *sequence, plasma_request_t *request);
+
 /*
  * Norms (reference compute/zlansy.c:81, :240; compute/zlange.c:76, :233;
  * compute/dzamax.c:21, :113).  norm: PlasmaMaxNorm, PlasmaOneNorm,

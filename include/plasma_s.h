@@ -62,6 +62,15 @@ void plasma_omp_str2desc(float *pA, int lda, plasma_desc_t A,
 void plasma_omp_sdesc2tr(plasma_desc_t A, float *pA, int lda,
                          plasma_sequence_t *sequence, plasma_request_t *request);
 
+/* Inverse of an SPD matrix (reference compute/zpotri.c:65, compute/
+ * zpoinv.c:67), single precision forms of plasma_d.h's. */
+int plasma_spotri(plasma_enum_t uplo, int n, float *pA, int lda);
+int plasma_spoinv(plasma_enum_t uplo, int n, float *pA, int lda);
+void plasma_omp_spotri(plasma_enum_t uplo, plasma_desc_t A,
+                       plasma_sequence_t *sequence, plasma_request_t *request);
+void plasma_omp_spoinv(plasma_enum_t uplo, plasma_desc_t A,
+                       plasma_sequence_t *sequence, plasma_request_t *request);
+
 /* Norms (reference compute/zlansy.c:81, :240; compute/zlange.c:76, :233;
  * compute/dzamax.c:21, :113), single precision forms of plasma_d.h's. */
 float plasma_slansy(plasma_enum_t norm, plasma_enum_t uplo, int n,

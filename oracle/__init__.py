@@ -39,6 +39,9 @@ def load():
         sig = {
             "ora_set_blas": (i, [i]), "ora_get_blas": (i, []),
             "ora_mkl_refused": (i, []),
+            "ora_dpotri": (i, [i, i, p, i, i]),
+            "ora_dpoinv": (i, [i, i, p, i, i]),
+            "ora_spotri": (i, [i, i, p, i, i]),
             "ora_dlansy": (d, [i, i, i, p, i]),
             "ora_dlange": (d, [i, i, i, p, i]),
             "ora_damax": (None, [i, i, i, p, i, p]),
@@ -107,6 +110,22 @@ def posv(uplo, a, b, nb, dtype=np.float64):
     info = fn(uplo, a.shape[0], b.shape[1], a.ctypes.data, _ld(a),
               b.ctypes.data, _ld(b), nb)
     return a, b, info
+
+
+def potri(uplo, f, nb, dtype=np.float64):
+    """A^{-1} in the uplo triangle from the Cholesky factor f (uplo
+    triangle; the other triangle passes through); (array, info)."""
+    a = _f(f, dtype).copy(order="F")
+    fn = load().ora_dpotri if dtype == np.float64 else load().ora_spotri
+    info = fn(uplo, a.shape[0], a.ctypes.data, _ld(a), nb)
+    return a, info
+
+
+def poinv(uplo, a, nb):
+    """A^{-1} in the uplo triangle from SPD A (factor, then potri)."""
+    a = _f(a, np.float64).copy(order="F")
+    info = load().ora_dpoinv(uplo, a.shape[0], a.ctypes.data, _ld(a), nb)
+    return a, info
 
 
 def potrs(uplo, l, b, nb):

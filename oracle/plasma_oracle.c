@@ -249,6 +249,18 @@ int ora_sposv(int uplo, int n, int nrhs, float *A, int lda, float *B,
 {
     return s_solve(uplo, n, nrhs, A, lda, B, ldb, nb, 1);
 }
+int ora_dpotri(int uplo, int n, double *A, int lda, int nb)
+{
+    return d_inverse(uplo, n, A, lda, nb, 0);
+}
+int ora_dpoinv(int uplo, int n, double *A, int lda, int nb)
+{
+    return d_inverse(uplo, n, A, lda, nb, 1);
+}
+int ora_spotri(int uplo, int n, float *A, int lda, int nb)
+{
+    return s_inverse(uplo, n, A, lda, nb, 0);
+}
 int ora_dpotrs(int uplo, int n, int nrhs, const double *A, int lda,
                double *B, int ldb, int nb)
 {

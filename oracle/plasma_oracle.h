@@ -33,6 +33,10 @@ enum { ORA_UPPER = 121, ORA_LOWER = 122 };
 int ora_set_blas(int backend);
 int ora_get_blas(void);
 int ora_mkl_refused(void);
+/* inverse from the factor / of A (reference compute/zpotri.c, zpoinv.c) */
+int ora_dpotri(int uplo, int n, double *A, int lda, int nb);
+int ora_dpoinv(int uplo, int n, double *A, int lda, int nb);
+int ora_spotri(int uplo, int n, float *A, int lda, int nb);
 /* norms (reference compute/zlansy.c, zlange.c, dzamax.c) */
 double ora_dlansy(int norm, int uplo, int n, const double *A, int lda);
 double ora_dlange(int norm, int m, int n, const double *A, int lda);

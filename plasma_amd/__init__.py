@@ -111,6 +111,10 @@ for _p, _t in (("d", "double"), ("s", "float")):
         "plasma_omp_%sdesc2ge" % _p: (None, [_D, c_void_p, c_int, _PS, _PR]),
         "plasma_omp_%str2desc" % _p: (None, [c_void_p, c_int, _D, _PS, _PR]),
         "plasma_omp_%sdesc2tr" % _p: (None, [_D, c_void_p, c_int, _PS, _PR]),
+        "plasma_%spotri" % _p: (c_int, [c_int, c_int, c_void_p, c_int]),
+        "plasma_%spoinv" % _p: (c_int, [c_int, c_int, c_void_p, c_int]),
+        "plasma_omp_%spotri" % _p: (None, [c_int, _D, _PS, _PR]),
+        "plasma_omp_%spoinv" % _p: (None, [c_int, _D, _PS, _PR]),
         "plasma_%slansy" % _p: (c_double if _p == "d" else ctypes.c_float,
                                 [c_int, c_int, c_int, c_void_p, c_int]),
         "plasma_%slange" % _p: (c_double if _p == "d" else ctypes.c_float,

@@ -168,6 +168,7 @@ void plasma_omp_xposv(plasma_enum_t precision, plasma_enum_t uplo,
     }
     if (uplo != PlasmaUpper && uplo != PlasmaLower) {
         plasma_error("illegal value of uplo");
+        plasma_request_fail(sequence, request, PlasmaErrorIllegalValue);
         return;
     }
     if (!check_pair(precision, A, B, sequence, request))

@@ -283,6 +283,16 @@ size_t plasma_lansy_work_doubles(plasma_desc_t A);
  * memory.  amax: per column (n values) or per row (mt * mb slots, the
  * first m meaningful) max |a_ij| into device values. */
 size_t plasma_norm_work_doubles(plasma_desc_t A);
+
+/* Pieces of the inverse from the factor (kernels/potri.hip): B (general)
+ * set to I; the first zero on the diagonal of the lower-equivalent factor
+ * A into *status as its 1-based index (if *status is 0); the uplo triangle
+ * of the symmetric Q (general) into A's stored tiles. */
+int plasma_kern_identity(plasma_desc_t B, const int *status, hipStream_t s);
+int plasma_kern_diag_zero(plasma_enum_t uplo, plasma_desc_t A, int *status,
+                          hipStream_t s);
+int plasma_kern_copy_tri(plasma_enum_t uplo, plasma_desc_t Q,
+                         plasma_desc_t A, const int *status, hipStream_t s);
 int plasma_kern_norm(plasma_enum_t norm, plasma_enum_t uplo, plasma_desc_t A,
                      double *work, double *value, const int *status,
                      hipStream_t s);

@@ -52,8 +52,9 @@ sys.path.insert(0, ROOT)
 
 FP64_MFMA_PEAK = 78.6e12      # MI355X dense FP64 matrix peak (FLOP/s)
 FP32_MFMA_PEAK = 157.3e12     # MI355X dense FP32 matrix peak (FLOP/s)
-BF16_MFMA_PEAK = 2.5e15       # MI355X dense bf16 matrix peak (FLOP/s)
-SPLIT_PRODUCTS = 6            # bf16 products per FP32 product (split update)
+HALF_MFMA_PEAK = 2.5e15       # MI355X dense bf16 / fp16 matrix peak (FLOP/s)
+# 16-bit MFMA products per FP32 product of the split updates
+SPLIT_PRODUCTS = {"bf16": 6, "fp16": 3}
 HBM_PEAK = 8.0e12             # bytes/s
 EPS64 = 2.0 ** -53
 
@@ -175,11 +176,20 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--batch", type=int, default=4096,
                     help="systems of the batched line (0 skips it)")
-    ap.add_argument("--adjoint-n", type=int, default=16384,
-                    help="order of the Cholesky-adjoint line (0 skips it)")
-    ap.add_argument("--split-update", choices=("on", "off"), default=None,
-                    help="FP32 trailing updates on bf16 MFMA with split "
-                         "operands (default: the library's setting)")
+    ap.add_argument("--adjoint-n", type=str, default="4096,16384",
+                    help="orders of the Cholesky-adjoint lines (empty "
+                         "skips them)")
+    ap.add_argument("--split-update", choices=("bf16", "fp16", "off"),
+                    default=None,
+                    help="FP32 trailing updates on 16-bit MFMA from exact "
+                         "operand splits: three bf16 terms (six products) "
+                         "or row-scaled two-term fp16 (three products); "
+                         "default: the library's setting")
+    ap.add_argument("--config2", type=str, default="8192,16384",
+                    help="orders of the plain FP64 dpotrf+dpotrs lines on "
+                         "G G^T/n + I (tuned nb; empty skips them)")
+    ap.add_argument("--no-host", action="store_true",
+                    help="skip the host-pointer (PCIe-inclusive) timings")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the solves, copy MIB MiB device to device "
                          "once (a kernel of known traffic for scaling "
@@ -187,13 +197,45 @@ def parse():
     return ap.parse_args()
 
 
+def host_cpu():
+    """Model, sockets and cores of this host (lscpu), the CPU share this
+    process may use, and the BLAS the oracle's tile kernels call."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, text=True,
+                             timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k in ("Model name", "Socket(s)", "Core(s) per socket",
+                     "Thread(s) per core", "CPU(s)"):
+                info[k] = v
+    except (OSError, subprocess.SubprocessError):
+        pass
+    info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    try:
+        lib = ctypes.CDLL("/opt/conda/lib/libmkl_rt.so.1")
+        buf = ctypes.create_string_buffer(256)
+        lib.mkl_get_version_string(buf, 256)
+        info["blas"] = buf.value.decode().strip()
+    except (OSError, AttributeError):
+        info["blas"] = "built-in loops"
+    return info
+
+
 def cpu_baseline(n, nrhs, nb, kappa, threads):
-    """Oracle dsposv on the host on the same kind of matrix (Householder,
-    cond = kappa) at order n: median of 2 runs after one warm-up."""
+    """The oracle (PLASMA's tile algorithms restated: OpenMP tasks,
+    sequential MKL tile kernels) on this host, on the same kind of matrices
+    at order n: dsposv on the Householder matrix of condition kappa and
+    plain dposv on G G^T/n + I; one warm-up and one timed run each (a
+    bounded sample, about 10-20 s)."""
     import numpy as np
     import oracle as ora
     backend = ora.set_blas(ora.MKL)
     if threads <= 0:
+        # the GPU box's CPU share for one GPU is 16 cores, whatever the
+        # affinity mask shows
         threads = min(16, len(os.sched_getaffinity(0)))
     ora.set_threads(threads)
     import torch
@@ -202,26 +244,38 @@ def cpu_baseline(n, nrhs, nb, kappa, threads):
     B = torch.randn(nrhs, n, dtype=torch.float64, device="cuda", generator=g)
     A = np.asfortranarray(A.cpu().numpy())
     B = np.asfortranarray(B.cpu().numpy().T)
-    times, iters = [], []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        _, it, info = ora.dsposv(ora.LOWER, A, B, nb)
-        times.append(time.perf_counter() - t0)
-        iters.append(it)
-        assert info == 0
-    t = sorted(times[1:])[len(times[1:]) // 2]
+    G = np.asfortranarray(ggt_spd(n, seed=3).cpu().numpy())
+    res = {}
+    for name in ("dsposv", "dposv"):
+        ts = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            if name == "dsposv":
+                _, it, info = ora.dsposv(ora.LOWER, A, B, nb)
+            else:
+                _, _, info = ora.posv(ora.LOWER, G, B, nb)
+            ts.append(time.perf_counter() - t0)
+            assert info == 0
+        res[name] = (ts[-1], it if name == "dsposv" else None)
+    t, iters = res["dsposv"]
+    tp, _ = res["dposv"]
+    cpu = host_cpu()
     return {
         "value": flops_posv(n, nrhs) / t / 1e12,
         "unit": "TFLOP/s",
         "cores": threads,
         "kind": "port",
         "sample": "oracle dsposv n=%d nrhs=%d nb=%d lower, Householder "
-                  "A cond %.0e, OpenMP tile tasks, %s sequential tile "
-                  "kernels, median of 2 runs after 1 warm-up (%.2f s per "
-                  "solve, %d refinement iterations)"
-                  % (n, nrhs, nb, kappa,
-                     "MKL" if backend == ora.MKL else "built-in", t,
-                     iters[-1]),
+                  "A cond %.0e, OpenMP tile tasks on %d threads, %s "
+                  "sequential tile kernels, one run after a warm-up "
+                  "(%.2f s per solve, %d refinement iterations)"
+                  % (n, nrhs, nb, kappa, threads,
+                     "MKL" if backend == ora.MKL else "built-in", t, iters),
+        "plain_dposv": {"value": flops_posv(n, nrhs) / tp / 1e12,
+                        "unit": "TFLOP/s", "seconds": round(tp, 3),
+                        "workload": "oracle dposv n=%d nrhs=%d nb=%d on "
+                                    "G G^T/n + I" % (n, nrhs, nb)},
+        "host": cpu,
     }
 
 
@@ -236,17 +290,22 @@ class Bench:
             raise SystemExit("plasma_init failed: %d" % rc)
         if args.split_update is not None:
             pa.set_param(pa.PlasmaSplitUpdate,
-                         pa.PlasmaEnabled if args.split_update == "on"
-                         else pa.PlasmaDisabled)
+                         {"bf16": pa.PlasmaSplitBf16,
+                          "fp16": pa.PlasmaSplitFp16,
+                          "off": pa.PlasmaDisabled}[args.split_update])
         if args.nb > 0:
             pa.set_param(pa.PlasmaTuning, pa.PlasmaDisabled)
             pa.set_param(pa.PlasmaNb, args.nb)
             pa.set_param(pa.PlasmaLookahead, args.lookahead)
             pa.set_param(pa.PlasmaUpdateDepth, 2)
 
-    def split_on(self):
-        return self.pa.get_param(self.pa.PlasmaSplitUpdate)[1] == \
-            self.pa.PlasmaEnabled
+    def split_mode(self):
+        """The FP32 update the library runs: "bf16", "fp16" or None (FP32
+        MFMA); plasma_get reports the split in force."""
+        pa = self.pa
+        v = pa.get_param(pa.PlasmaSplitUpdate)[1]
+        return {pa.PlasmaSplitFp16: "fp16",
+                pa.PlasmaSplitBf16: "bf16"}.get(v)
 
     def schedule(self, prec, n):
         """(nb, update depth, lookahead) of a factorization of order n: the
@@ -369,6 +428,79 @@ class Bench:
         return ms, prof, res, iters[-1]
 
 
+def host_line(bench, A0, B0, kappa):
+    """The drop-in rate with host arrays (PCIe-inclusive): plasma_dposv and
+    plasma_dsposv on page-locked host copies of the headline system, the
+    library staging A and B through HBM.  One warm-up, then the mean of
+    `steps` calls (the input copied back between calls, untimed)."""
+    import torch
+    pa, lib, steps = bench.pa, bench.lib, bench.args.steps
+    n, nrhs = A0.shape[0], B0.shape[0]
+    hA0 = torch.empty(A0.shape, dtype=A0.dtype, pin_memory=True)
+    hB0 = torch.empty(B0.shape, dtype=B0.dtype, pin_memory=True)
+    hA0.copy_(A0)
+    hB0.copy_(B0)
+    hA = torch.empty_like(hA0, pin_memory=True)
+    hB = torch.empty_like(hB0, pin_memory=True)
+    hX = torch.empty_like(hB0, pin_memory=True)
+    it = ctypes.c_int(0)
+    out = {"workload": "headline system (cond %.0e) with page-locked host "
+                       "arrays: the library copies A and B to HBM and the "
+                       "results back inside the timed call" % kappa}
+    for name in ("plain", "mixed"):
+        ts = []
+        for rep in range(steps + 1):
+            hA.copy_(hA0)
+            hB.copy_(hB0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if name == "plain":
+                rc = lib.plasma_dposv(pa.PlasmaLower, n, nrhs, hA.data_ptr(),
+                                      n, hB.data_ptr(), n)
+            else:
+                rc = lib.plasma_dsposv(pa.PlasmaLower, n, nrhs,
+                                       hA.data_ptr(), n, hB.data_ptr(), n,
+                                       hX.data_ptr(), n, ctypes.byref(it))
+            ts.append(time.perf_counter() - t0)
+            if rc != 0:
+                raise SystemExit("host-pointer %s solve failed: %d"
+                                 % (name, rc))
+        t = sum(ts[1:]) / steps
+        out[name + "_ms"] = round(t * 1e3, 2)
+        out[name + "_tflops"] = round(flops_posv(n, nrhs) / t / 1e12, 3)
+        print("bench: host-pointer %s done" % name, file=sys.stderr,
+              flush=True)
+    del hA0, hB0, hA, hB, hX
+    return out
+
+
+def config2_line(bench, orders, nrhs):
+    """Plain FP64 plasma_dposv (dpotrf + dpotrs) on A = G G^T/n + I at the
+    given orders, autotuned schedule, graph mode: rate against the FP64
+    matrix peak."""
+    import torch
+    out = {}
+    g = torch.Generator(device="cuda").manual_seed(7)
+    for n in orders:
+        A0 = ggt_spd(n, seed=n)
+        B0 = torch.randn(nrhs, n, dtype=torch.float64, device="cuda",
+                         generator=g)
+        torch.cuda.synchronize()
+        bench.pa.set_param(bench.pa.PlasmaGraphCapture,
+                           bench.pa.PlasmaEnabled)
+        ms, _, res = bench.plain(A0, B0, "config2 n=%d" % n, profile=False)
+        tf = flops_posv(n, nrhs) / (ms * 1e-3)
+        out["n=%d" % n] = {
+            "ms": round(ms, 3), "tflops": round(tf / 1e12, 3),
+            "frac_fp64_peak": round(tf / FP64_MFMA_PEAK, 4),
+            "residual_eps": round(res, 4),
+            "schedule": dict(zip(("nb", "update_depth", "lookahead"),
+                                 bench.schedule("d", n)))}
+        del A0, B0
+        torch.cuda.empty_cache()
+    return out
+
+
 def classes(prof, steps):
     out = {}
     for name, p in prof.items():
@@ -425,9 +557,18 @@ def batched_line(bench, count, steps):
             res = (r.abs().max() / (k * M.abs().sum(1).max() *
                                     x.abs().max())).item() / eps
             worst = max(worst, res)
+        # algorithmic traffic: each system's stored triangle read and its
+        # factor written (n^2 elements), its right-hand side read and
+        # written
+        es = 8 if prec == "d" else 4
+        nbytes = float(((nf ** 2) + 2 * nf).sum()) * es
+        peak = FP64_MFMA_PEAK if prec == "d" else FP32_MFMA_PEAK
         out[{"d": "fp64", "s": "fp32"}[prec]] = {
             "ms": round(t * 1e3, 3),
             "tflops": round(flops / t / 1e12, 3),
+            "frac_mfma_peak": round(flops / t / peak, 4),
+            "gbps": round(nbytes / t / 1e9, 1),
+            "frac_hbm": round(nbytes / t / HBM_PEAK, 4),
             "max_residual_eps": round(worst, 4)}
         del A, B
     return out
@@ -546,21 +687,33 @@ def roofline(prof, kernel, peak, traffic=None):
 
 def split_roofline(r, split):
     """The FP32 update's roofline entry: FP32-algorithmic flops against the
-    FP32 matrix peak; with the split update (bf16 MFMA, six products per
-    FP32 product) also the bf16 MFMA work against the bf16 peak."""
+    FP32 matrix peak; with a split update also the 16-bit MFMA work it
+    issues (six bf16 or three fp16 products per FP32 product) against the
+    16-bit dense peak."""
     if r is None:
         return None
     if not split:
         r["kernel"] += ": tg::kernel<float,...,UpdateDecode> on FP32 MFMA"
         return r
-    r["kernel"] += (": split_update_kernel, operands split into three bf16 "
-                    "terms, v_mfma_f32_16x16x32_bf16")
-    work = r["achieved"] * 1e12 * SPLIT_PRODUCTS
-    r["mfma_work"] = {"unit": "TFLOP/s", "products_per_fp32_product":
-                      SPLIT_PRODUCTS, "achieved": round(work / 1e12, 1),
-                      "peak": BF16_MFMA_PEAK / 1e12,
-                      "frac": round(work / BF16_MFMA_PEAK, 4)}
+    if split == "bf16":
+        r["kernel"] += (": split_update_kernel, operands split into three "
+                        "bf16 terms, v_mfma_f32_16x16x32_bf16")
+    else:
+        r["kernel"] += (": split16_update_kernel, row-scaled operands split "
+                        "into two fp16 terms, global_load_lds_dwordx4 "
+                        "fragments, v_mfma_f32_16x16x32_f16")
+    products = SPLIT_PRODUCTS[split]
+    work = r["achieved"] * 1e12 * products
+    r["mfma_work"] = {"unit": "TFLOP/s", "precision": split,
+                      "products_per_fp32_product": products,
+                      "achieved": round(work / 1e12, 1),
+                      "peak": HALF_MFMA_PEAK / 1e12,
+                      "frac": round(work / HALF_MFMA_PEAK, 4)}
     return r
+
+
+def orders(text):
+    return [int(x) for x in text.split(",") if x.strip()]
 
 
 def main():
@@ -581,6 +734,7 @@ def main():
 
     sweep = {}
     head = None
+    host = None
     for kappa in kappas:
         A0, _ = householder_spd(n, kappa, seed=0)
         torch.cuda.synchronize()
@@ -592,16 +746,21 @@ def main():
         ms_m, _, res_m, iters = bench.mixed(A0, B0, "mixed " + tag,
                                             profile=False)
         ms_p, _, res_p = bench.plain(A0, B0, "plain " + tag, profile=False)
-        # the bf16-input factor with GMRES-IR on the same system
-        ms_h, _, res_h, it_h = bench.mixed(A0, B0, "bf16 gmres-ir " + tag,
-                                           profile=False,
-                                           routine="plasma_dhposv")
+        # the half-precision-input factors with GMRES-IR on the same system
+        half = {}
+        for fmt, val in (("bf16", bench.pa.PlasmaSplitBf16),
+                         ("fp16", bench.pa.PlasmaSplitFp16)):
+            bench.pa.set_param(bench.pa.PlasmaHalfInput, val)
+            half[fmt] = bench.mixed(A0, B0, "%s gmres-ir %s" % (fmt, tag),
+                                    profile=False, routine="plasma_dhposv")
         # profiled: the same solves and schedules (graph capture stays
         # enabled, so the library picks the same rows of its table) launched
         # one by one, since profiling bypasses the graphs; every launch
         # bracketed by HIP events (kernel classes and the roofline)
         pms_m, prof_m, _, _ = bench.mixed(A0, B0, "mixed profiled " + tag)
         pms_p, prof_p, _ = bench.plain(A0, B0, "plain profiled " + tag)
+        if kappa == kappas[-1] and not args.no_host:
+            host = host_line(bench, A0, B0, kappa)
         del A0
         sweep[tag] = {
             "mixed_tflops": round(fl / (ms_m * 1e-3) / 1e12, 3),
@@ -614,11 +773,13 @@ def main():
             "speedup": round(ms_p / ms_m, 3),
             "profiled_mixed_ms": round(pms_m, 3),
             "profiled_plain_ms": round(pms_p, 3),
-            "bf16_gmres_ms": round(ms_h, 3),
-            "bf16_gmres_tflops": round(fl / (ms_h * 1e-3) / 1e12, 3),
-            "bf16_gmres_inner_iterations": it_h,
-            "bf16_gmres_residual_eps": round(res_h, 4),
         }
+        for fmt, (ms_h, _, res_h, it_h) in half.items():
+            sweep[tag].update({
+                fmt + "_gmres_ms": round(ms_h, 3),
+                fmt + "_gmres_tflops": round(fl / (ms_h * 1e-3) / 1e12, 3),
+                fmt + "_gmres_inner_iterations": it_h,
+                fmt + "_gmres_residual_eps": round(res_h, 4)})
         head = (kappa, ms_m, prof_m, res_m, iters, ms_p, prof_p, res_p,
                 pms_m, pms_p)
         print("bench: %s %s" % (tag, json.dumps(sweep[tag])),
@@ -692,13 +853,15 @@ def main():
                      FP32_MFMA_PEAK,
                      pmc_traffic("trailing_update_fp32_dsposv_"
                                  "n32768_pmc_traffic")),
-            bench.split_on()),
+            bench.split_mode()),
         "kernels": classes(prof_m, args.steps),
         "plain_ggt": ggt,
         "batched": batched_line(bench, args.batch, args.steps)
         if args.batch > 0 else None,
-        "adjoint": adjoint_line(bench, args.adjoint_n, args.steps)
-        if args.adjoint_n > 0 else None,
+        "adjoint": {"n=%d" % k: adjoint_line(bench, k, args.steps)
+                    for k in orders(args.adjoint_n)} or None,
+        "config2": config2_line(bench, orders(args.config2), nrhs) or None,
+        "host_pointers": host,
     }
     if args.calibrate_copy > 0:
         nel = args.calibrate_copy * (1 << 20) // 8

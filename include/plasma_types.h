@@ -129,7 +129,8 @@ enum {
  *                    at most 1 in magnitude, two fp16 terms, three products
  *                    (22-bit operands, FP32 accumulation);
  *   PlasmaSplitBf16: three bf16 terms, six products (FP32-exact products);
- *   PlasmaEnabled:   the default split (PlasmaSplitBf16);
+ *   PlasmaEnabled:   the default split (PlasmaSplitBf16; plasma_get
+ *                    returns the split in force);
  *   PlasmaDisabled:  FP32 MFMA.
  * Values of PlasmaHalfInput: PlasmaSplitBf16 (default) or PlasmaSplitFp16,
  * the one-term input format of plasma_dhposv's low-precision factor.

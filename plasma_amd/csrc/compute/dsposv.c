@@ -70,6 +70,48 @@ static void solve_s(plasma_enum_t uplo, plasma_desc_t As, plasma_desc_t Xs,
         plasma_pxpotrs(uplo, As, Xs, inv_s, sequence, request);
 }
 
+/* R = B - A X, the column max-norms of X and R and the convergence flag
+ * (reference zcposv.c:466-476 and :509-516), one profiler record (class
+ * mixed: the residual's 2 n^2 nrhs flops and its read of A's triangle). */
+static int residual_step(plasma_enum_t uplo, plasma_desc_t A,
+                         plasma_desc_t B, plasma_desc_t X, plasma_desc_t R,
+                         void *symm_work, size_t symm_bytes, double *xnorm,
+                         double *rnorm, const double *anorm, double eps_sqrtn,
+                         int *dflag, int *st, hipStream_t s,
+                         plasma_sequence_t *sequence,
+                         plasma_request_t *request)
+{
+    const double n = A.n, nrhs = X.n;
+    int slot;
+    plasma_prof_begin(s, PlasmaProfMixed, 2.0 * n * n * nrhs,
+                      n * (n + 1) / 2 * 8.0 + 3.0 * n * nrhs * 8.0, &slot);
+    int err =
+        issue(plasma_kern_lacpy(B, R, st, s), sequence, request) ||
+        issue(plasma_kern_symm_residual(uplo, A, X, R, symm_work, symm_bytes,
+                                        st, s), sequence, request) ||
+        issue(plasma_kern_amax_cols(X, xnorm, st, s), sequence, request) ||
+        issue(plasma_kern_amax_cols(R, rnorm, st, s), sequence, request) ||
+        issue(plasma_kern_conv_test(rnorm, xnorm, (int)nrhs, anorm,
+                                    eps_sqrtn, dflag, st, s),
+              sequence, request);
+    plasma_prof_end(s, slot);
+    return err;
+}
+
+/* FP64 <-> FP32 conversion of a descriptor, one profiler record (class
+ * mixed: 12 bytes per element). */
+static int lag2_step(plasma_desc_t S, plasma_desc_t D, int *st, hipStream_t s,
+                     plasma_sequence_t *sequence, plasma_request_t *request)
+{
+    int slot;
+    plasma_prof_begin(s, PlasmaProfMixed, 0,
+                      (double)plasma_desc_storage_bytes(S) +
+                          (double)plasma_desc_storage_bytes(D), &slot);
+    int err = issue(plasma_kern_lag2(S, D, st, s), sequence, request);
+    plasma_prof_end(s, slot);
+    return err;
+}
+
 void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
                        plasma_desc_t X, plasma_desc_t As, plasma_desc_t Xs,
                        plasma_desc_t R, double *W, double *Rnorm,
@@ -154,11 +196,17 @@ void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
     int conv;
 
     /* reference zcposv.c:442-463 */
-    if (issue(plasma_kern_lansy_inf(uplo, A, work, anorm, st, s), sequence,
-              request) ||
-        issue(plasma_kern_lag2(B, Xs, st, s), sequence, request) ||
-        issue(plasma_kern_lag2(A, As, st, s), sequence, request))
-        goto done;
+    {
+        int slot;
+        plasma_prof_begin(s, PlasmaProfMixed, 0,
+                          (double)plasma_desc_storage_bytes(A), &slot);
+        const int err = issue(plasma_kern_lansy_inf(uplo, A, work, anorm, st,
+                                                    s), sequence, request);
+        plasma_prof_end(s, slot);
+        if (err || lag2_step(B, Xs, st, s, sequence, request) ||
+            lag2_step(A, As, st, s, sequence, request))
+            goto done;
+    }
     plasma_pxpotrf(uplo, As, inv_s, sequence, request);
     /* the refinement solves with this factor up to 31 times: scale the
      * panels once so that every sweep step is one launch */
@@ -166,18 +214,12 @@ void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
     if (mpan != NULL)
         plasma_pxscale_panels(uplo, As, inv_s, mpan, sequence, request);
     solve_s(uplo, As, Xs, inv_s, mpan, sequence, request);
-    if (issue(plasma_kern_lag2(Xs, X, st, s), sequence, request))
+    if (lag2_step(Xs, X, st, s, sequence, request))
         goto done;
 
     /* R = B - A X and the test (reference :466-481) */
-    if (issue(plasma_kern_lacpy(B, R, st, s), sequence, request) ||
-        issue(plasma_kern_symm_residual(uplo, A, X, R, symm_work,
-                                            symm_bytes, st, s), sequence,
-              request) ||
-        issue(plasma_kern_amax_cols(X, xnorm, st, s), sequence, request) ||
-        issue(plasma_kern_amax_cols(R, rnorm, st, s), sequence, request) ||
-        issue(plasma_kern_conv_test(rnorm, xnorm, nrhs, anorm, eps_sqrtn,
-                                    dflag, st, s), sequence, request))
+    if (residual_step(uplo, A, B, X, R, symm_work, symm_bytes, xnorm, rnorm,
+                      anorm, eps_sqrtn, dflag, st, s, sequence, request))
         goto done;
     conv = read_conv(ctx, dflag, st, sequence, request);
     if (conv != 0)
@@ -185,23 +227,15 @@ void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
 
     /* refinement (reference :485-518) */
     for (int it = 0; it < itermax; it++) {
-        if (issue(plasma_kern_lag2(R, Xs, st, s), sequence, request))
+        if (lag2_step(R, Xs, st, s, sequence, request))
             goto done;
         solve_s(uplo, As, Xs, inv_s, mpan, sequence, request);
-        if (issue(plasma_kern_lag2(Xs, R, st, s), sequence, request) ||
+        if (lag2_step(Xs, R, st, s, sequence, request) ||
             issue(plasma_kern_geadd(1.0, R, 1.0, X, st, s), sequence,
                   request) ||
-            issue(plasma_kern_lacpy(B, R, st, s), sequence, request) ||
-            issue(plasma_kern_symm_residual(uplo, A, X, R, symm_work,
-                                            symm_bytes, st, s), sequence,
-                  request) ||
-            issue(plasma_kern_amax_cols(X, xnorm, st, s), sequence,
-                  request) ||
-            issue(plasma_kern_amax_cols(R, rnorm, st, s), sequence,
-                  request) ||
-            issue(plasma_kern_conv_test(rnorm, xnorm, nrhs, anorm,
-                                        eps_sqrtn, dflag, st, s),
-                  sequence, request))
+            residual_step(uplo, A, B, X, R, symm_work, symm_bytes, xnorm,
+                          rnorm, anorm, eps_sqrtn, dflag, st, s, sequence,
+                          request))
             goto done;
         conv = read_conv(ctx, dflag, st, sequence, request);
         if (conv < 0)

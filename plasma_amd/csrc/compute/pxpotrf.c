@@ -109,8 +109,8 @@ typedef struct {
 /* FP32 factorizations whose tiles are a multiple of 32 run their trailing
  * updates on the 16-bit matrix cores: the low-precision input factor of the
  * GMRES-IR solver takes one product of bf16 or fp16 inputs; otherwise
- * PlasmaSplitUpdate chooses the split (PlasmaEnabled: the default, the
- * three-term bf16 form) or FP32 MFMA (PlasmaDisabled). */
+ * PlasmaSplitUpdate chooses the split (PlasmaEnabled selects
+ * PLASMA_DEFAULT_SPLIT when set) or FP32 MFMA (PlasmaDisabled). */
 static int use_split(plasma_desc_t A)
 {
     plasma_context_t *ctx = plasma_context_self();
@@ -121,7 +121,6 @@ static int use_split(plasma_desc_t A)
     if (ctx->low_factor == PlasmaLowFp16)
         return SPLIT_FP16X1;
     switch (ctx->split_update) {
-    case PlasmaEnabled:
     case PlasmaSplitBf16:  return SPLIT_BF16X3;
     case PlasmaSplitFp16:  return SPLIT_FP16X2;
     default:               return SPLIT_NONE;
@@ -696,8 +695,15 @@ void plasma_pxscale_panels(plasma_enum_t uplo, plasma_desc_t A,
         return;
     plasma_context_t *ctx = plasma_context_self();
     int *st = plasma_sequence_dstatus(sequence, request);
+    /* 2 mv nb^2 flops per strictly lower tile (M(m,k) = L(m,k) W_kk) */
+    double flops = 0;
+    for (int k = 0; k + 1 < A.mt; k++)
+        flops += 2.0 * A.nb * A.nb * (A.m - (k + 1) * A.mb);
+    int slot;
+    plasma_prof_begin(ctx->stream, PlasmaProfSweep, flops, 0, &slot);
     issue(plasma_kern_scale_panels(uplo, A, inv, M, st, ctx->stream),
           sequence, request);
+    plasma_prof_end(ctx->stream, slot);
 }
 
 void plasma_pxpotrs_m(plasma_desc_t A, plasma_desc_t B, const void *inv,

@@ -58,7 +58,7 @@ static void context_defaults(plasma_context_t *ctx)
     ctx->householder_mode = PlasmaFlatHouseholder;
     ctx->lookahead = 1;
     ctx->group = 2;
-    ctx->split_update = PlasmaEnabled;
+    ctx->split_update = PLASMA_DEFAULT_SPLIT;
     ctx->half_input = PlasmaSplitBf16;
     ctx->low_factor = 0;
     ctx->graph_capture = PlasmaDisabled;
@@ -212,7 +212,8 @@ int plasma_set(plasma_enum_t param, int value)
             plasma_error("invalid split update mode");
             return PlasmaErrorIllegalValue;
         }
-        ctx->split_update = value;
+        ctx->split_update = value == PlasmaEnabled ? PLASMA_DEFAULT_SPLIT
+                                                   : value;
         break;
     case PlasmaHalfInput:
         if (value != PlasmaSplitBf16 && value != PlasmaSplitFp16) {

@@ -8,17 +8,21 @@
  * absent.  Here the table is plain text, written by the tuner
  * (plasma_amd/tune.py) from measurements on the GPU:
  *
- *     # routine precision n_max nb update_depth lookahead [mode]
+ *     # routine precision n_max nb update_depth lookahead [mode [split]]
  *     potrf d 8192 256 2 1 graph
+ *     potrf s 32768 256 4 2 graph fp16
  *
- * The optional mode, "eager" or "graph", says which launch mode the row was
- * measured in (PlasmaGraphCapture disabled or enabled); a row without one
- * serves both.  A factorization of order n in precision p uses, among the
- * rows of p for the context's mode (those of the other mode when p has
- * none), the first with n <= n_max (rows sorted by n_max), or the last.
- * Like the
- * reference, a tuned call leaves its choice in the context (nb, update
- * depth, lookahead).  The table is read at plasma_init from
+ * The optional mode, "eager", "graph" or "any", says which launch mode the
+ * row was measured in (PlasmaGraphCapture disabled or enabled); "any" or no
+ * mode serves both.  An FP32 row may also name the operands of its trailing
+ * updates, the MFMA form tuned beside the schedule: "fp32" (FP32 MFMA),
+ * "bf16" (three-term bf16 split, six products) or "fp16" (row-scaled
+ * two-term fp16, three products).  A factorization of order n in precision
+ * p uses, among the rows of p for the context's mode (those of the other
+ * mode when p has none), the first with n <= n_max (rows sorted by n_max),
+ * or the last.  Like the reference, a tuned call leaves its choice in the
+ * context (nb, update depth, lookahead, and the split when the row names
+ * one).  The table is read at plasma_init from
  * $PLASMA_TUNING_FILE, else from tuning/mi355x.txt next to the library's
  * directory; plasma_tuning_load() replaces it.
  */
@@ -53,22 +57,35 @@ int plasma_tuning_load(const char *path)
     char line[256];
     while (fgets(line, sizeof(line), f) != NULL &&
            ctx->ntune < PLASMA_TUNE_ROWS) {
-        char routine[32], prec, mode[16] = "";
+        char routine[32], prec, mode[16] = "", split[16] = "";
         int n_max, nb, depth, la;
-        const int got = sscanf(line, "%31s %c %d %d %d %d %15s", routine,
-                               &prec, &n_max, &nb, &depth, &la, mode);
+        const int got = sscanf(line, "%31s %c %d %d %d %d %15s %15s",
+                               routine, &prec, &n_max, &nb, &depth, &la, mode,
+                               split);
         if (line[0] == '#' || got < 6 || strcmp(routine, "potrf") != 0)
             continue;
         if ((prec != 'd' && prec != 's') || n_max < 1 || nb < 1 ||
             depth < 1 || depth > 4 || la < 0 || la > 4)
             continue;
         int m = 0;
-        if (got == 7 && strcmp(mode, "eager") == 0)
+        if (got >= 7 && strcmp(mode, "eager") == 0)
             m = 1;
-        else if (got == 7 && strcmp(mode, "graph") == 0)
+        else if (got >= 7 && strcmp(mode, "graph") == 0)
             m = 2;
-        else if (got == 7)
+        else if (got >= 7 && strcmp(mode, "any") != 0)
             continue;
+        /* FP32 rows may name the trailing update's operand format */
+        int sp = 0;
+        if (got == 8) {
+            if (strcmp(split, "fp32") == 0)
+                sp = -1;
+            else if (strcmp(split, "bf16") == 0)
+                sp = PlasmaSplitBf16;
+            else if (strcmp(split, "fp16") == 0)
+                sp = PlasmaSplitFp16;
+            else
+                continue;
+        }
         plasma_tune_row_t *r = &ctx->tune[ctx->ntune++];
         r->precision = prec == 'd' ? PlasmaRealDouble : PlasmaRealFloat;
         r->n_max = n_max;
@@ -76,6 +93,7 @@ int plasma_tuning_load(const char *path)
         r->depth = depth;
         r->lookahead = la;
         r->mode = m;
+        r->split = sp;
     }
     fclose(f);
     qsort(ctx->tune, ctx->ntune, sizeof(ctx->tune[0]), row_cmp);
@@ -123,4 +141,6 @@ void plasma_tune_potrf(plasma_context_t *ctx, plasma_enum_t precision, int n)
     ctx->nb = pick->nb;
     ctx->group = pick->depth;
     ctx->lookahead = pick->lookahead;
+    if (pick->split != 0)
+        ctx->split_update = pick->split < 0 ? PlasmaDisabled : pick->split;
 }

@@ -40,6 +40,8 @@ typedef struct {
 typedef struct {
     int precision, n_max, nb, depth, lookahead;
     int mode;                 /* 0 any, 1 eager launches, 2 graph replay */
+    int split;                /* FP32 update operands: 0 as set, -1 FP32,
+                                 PlasmaSplitBf16, PlasmaSplitFp16 */
 } plasma_tune_row_t;
 
 /* Deepest panel lookahead (PlasmaLookahead): the panel stream runs up to
@@ -132,6 +134,10 @@ struct plasma_context_s {
 };
 
 #define PLASMA_SCALARS 64
+
+/* the split PlasmaSplitUpdate = PlasmaEnabled selects (plasma_get reports
+ * the mode in force) */
+#define PLASMA_DEFAULT_SPLIT PlasmaSplitBf16
 
 /* values of plasma_context_t.low_factor */
 #define PlasmaLowBf16 1

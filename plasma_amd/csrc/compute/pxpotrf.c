@@ -25,6 +25,7 @@
  * expressed, so the critical path potrf -> trsm -> next column overlaps the
  * bulk updates.  The streams are joined again before returning.
  */
+#include <stdlib.h>
 #include <string.h>
 
 #include "internal.h"
@@ -191,6 +192,20 @@ static int k_panel(const group_t *g, int k, void *panel, int pbase,
     return err;
 }
 
+/* FP64 trailing updates with their operands loaded straight into LDS
+ * (kernels/update64.hip) unless PLASMA_FP64_UPDATE=staged selects the
+ * register-staged MFMA kernel (kept for comparison and for tile sizes that
+ * are not a multiple of 128). */
+static int update64_on(void)
+{
+    static int flag = -1;
+    if (flag < 0) {
+        const char *e = getenv("PLASMA_FP64_UPDATE");
+        flag = !(e != NULL && strcmp(e, "staged") == 0);
+    }
+    return flag;
+}
+
 static int k_update(const group_t *g, int pbase, int ncols, int n0, int n1,
                     const void *panel, int copy_cols, int max_wgs,
                     hipStream_t s)
@@ -211,6 +226,10 @@ static int k_update(const group_t *g, int pbase, int ncols, int n0, int n1,
         err = plasma_kern_trailing_update_split(
             g->uplo, g->A, pbase, kdepth, n0, n1, panel, planes_of(g, panel),
             g->pcols, copy_cols, max_wgs, g->split == SPLIT_BF16X1, g->st, s);
+    else if (update64_on() && plasma_update64_ok(g->A, kdepth))
+        err = plasma_kern_trailing_update64(g->uplo, g->A, pbase, kdepth, n0,
+                                            n1, panel, g->pcols, copy_cols,
+                                            max_wgs, g->st, s);
     else
         err = plasma_kern_trailing_update(g->uplo, g->A, pbase, kdepth, n0,
                                           n1, panel, g->pcols, copy_cols,

@@ -235,6 +235,17 @@ int plasma_kern_trailing_update_split(plasma_enum_t uplo, plasma_desc_t A,
                                       int copy_cols, int max_wgs, int single,
                                       const int *status, hipStream_t s);
 
+/* FP64 trailing update with operands loaded straight into LDS
+ * (kernels/update64.hip): same arguments and results as
+ * plasma_kern_trailing_update; plasma_update64_ok(A, kdepth) tells whether
+ * it applies (FP64, square tiles, mb a multiple of 128, kdepth of 16). */
+int plasma_update64_ok(plasma_desc_t A, int kdepth);
+int plasma_kern_trailing_update64(plasma_enum_t uplo, plasma_desc_t A,
+                                  int pbase, int kdepth, int n_first,
+                                  int n_end, const void *panel, int pcols,
+                                  int copy_cols, int max_wgs,
+                                  const int *status, hipStream_t s);
+
 /* FP32 trailing updates on fp16 MFMA (kernels/split16.hip).  Rows are
  * scaled by sigma_r = 2^-ceil(log2 sqrt(a_rr)) (plasma_kern_split16_scale,
  * plasma_split16_scale_elems(A) floats, from the diagonal of A before the

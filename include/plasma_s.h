@@ -88,6 +88,37 @@ void plasma_omp_samax(plasma_enum_t colrow, plasma_desc_t A, double *work,
                       float *values, plasma_sequence_t *sequence,
                       plasma_request_t *request);
 
+/*
+ * BLAS-3 (reference compute/zgemm.c:105, :315; compute/zsyrk.c:89, :263;
+ * compute/ztrsm.c:104, :290).  C = alpha op(A) op(B) + beta C;
+ * C = alpha A A^T + beta C (NoTrans) or alpha A^T A + beta C (Trans) in the
+ * uplo triangle of C; B = alpha op(A)^{-1} B (Left) or alpha B op(A)^{-1}
+ * (Right) with A triangular (uplo, diag).  PlasmaConjTrans is PlasmaTrans.
+ * Return codes as the reference (gemm: -1 transa, -2 transb, -3 m, -4 n,
+ * -5 k, -8 lda, -10 ldb, -13 ldc; syrk: -1 uplo, -2 trans, -3 n, -4 k,
+ * -7 lda, -10 ldc; trsm: -1 side, -2 uplo, -3 transa, -4 diag, -5 m, -6 n,
+ * -8 lda, -10 ldb).  Tile forms: descriptors of one square tile size.
+ */
+int plasma_sgemm(plasma_enum_t transa, plasma_enum_t transb, int m, int n,
+                 int k, float alpha, float *pA, int lda, float *pB, int ldb,
+                 float beta, float *pC, int ldc);
+void plasma_omp_sgemm(plasma_enum_t transa, plasma_enum_t transb,
+                      float alpha, plasma_desc_t A, plasma_desc_t B,
+                      float beta, plasma_desc_t C, plasma_sequence_t *sequence,
+                      plasma_request_t *request);
+int plasma_ssyrk(plasma_enum_t uplo, plasma_enum_t trans, int n, int k,
+                 float alpha, float *pA, int lda, float beta, float *pC, int ldc);
+void plasma_omp_ssyrk(plasma_enum_t uplo, plasma_enum_t trans, float alpha,
+                      plasma_desc_t A, float beta, plasma_desc_t C,
+                      plasma_sequence_t *sequence, plasma_request_t *request);
+int plasma_strsm(plasma_enum_t side, plasma_enum_t uplo,
+                 plasma_enum_t transa, plasma_enum_t diag, int m, int n,
+                 float alpha, float *pA, int lda, float *pB, int ldb);
+void plasma_omp_strsm(plasma_enum_t side, plasma_enum_t uplo,
+                      plasma_enum_t transa, plasma_enum_t diag, float alpha,
+                      plasma_desc_t A, plasma_desc_t B,
+                      plasma_sequence_t *sequence, plasma_request_t *request);
+
 #ifdef __cplusplus
 }
 #endif

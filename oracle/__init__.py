@@ -35,6 +35,7 @@ def load():
             build()
         lib = ctypes.CDLL(LIB_PATH)
         i, p, d = ctypes.c_int, ctypes.c_void_p, ctypes.c_double
+        f = ctypes.c_float
         pi = ctypes.POINTER(ctypes.c_int)
         sig = {
             "ora_set_blas": (i, [i]), "ora_get_blas": (i, []),
@@ -59,6 +60,11 @@ def load():
             "ora_tile_storage_elems": (ctypes.c_long, [i, i, i, i]),
             "ora_lapack2tile": (None, [i, i, i, p, i, p, i]),
             "ora_tile2lapack": (None, [i, i, i, p, p, i, i]),
+            "ora_dgemm": (i, [i, i, i, i, i, d, p, i, p, i, d, p, i, i]),
+            "ora_dsyrk": (i, [i, i, i, i, d, p, i, d, p, i, i]),
+            "ora_dtrsm": (i, [i, i, i, i, i, i, d, p, i, p, i, i]),
+            "ora_sgemm": (i, [i, i, i, i, i, f, p, i, p, i, f, p, i, i]),
+            "ora_strsm": (i, [i, i, i, i, i, i, f, p, i, p, i, i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(lib, name)
@@ -213,3 +219,48 @@ def amax(colrow, a):
 
 def plant_mistake(which):
     load().ora_plant_mistake(which)
+
+
+NOTRANS, TRANS = 111, 112
+LEFT, RIGHT = 141, 142
+NONUNIT, UNIT = 131, 132
+
+
+def gemm(ta, tb, alpha, a, b, beta, c, nb, dtype=np.float64):
+    """alpha op(a) op(b) + beta c by the tile algorithm (reference
+    compute/pzgemm.c); returns the new c."""
+    a, b = _f(a, dtype), _f(b, dtype)
+    c = _f(c, dtype).copy(order="F")
+    m, n = c.shape
+    k = a.shape[1] if ta == NOTRANS else a.shape[0]
+    fn = load().ora_dgemm if dtype == np.float64 else load().ora_sgemm
+    rc = fn(ta, tb, m, n, k, alpha, a.ctypes.data, _ld(a), b.ctypes.data,
+            _ld(b), beta, c.ctypes.data, _ld(c), nb)
+    assert rc == 0, rc
+    return c
+
+
+def syrk(uplo, trans, alpha, a, beta, c, nb):
+    """alpha a a^T + beta c (NOTRANS) or alpha a^T a + beta c (TRANS) in the
+    uplo triangle of c (reference compute/pzsyrk.c); returns the new c."""
+    a = _f(a, np.float64)
+    c = _f(c, np.float64).copy(order="F")
+    n = c.shape[0]
+    k = a.shape[1] if trans == NOTRANS else a.shape[0]
+    rc = load().ora_dsyrk(uplo, trans, n, k, alpha, a.ctypes.data, _ld(a),
+                          beta, c.ctypes.data, _ld(c), nb)
+    assert rc == 0, rc
+    return c
+
+
+def trsm(side, uplo, trans, diag, alpha, a, b, nb, dtype=np.float64):
+    """alpha op(a)^-1 b (LEFT) or alpha b op(a)^-1 (RIGHT) by the tile
+    algorithm (reference compute/pztrsm.c); returns the new b."""
+    a = _f(a, dtype)
+    b = _f(b, dtype).copy(order="F")
+    m, n = b.shape
+    fn = load().ora_dtrsm if dtype == np.float64 else load().ora_strsm
+    rc = fn(side, uplo, trans, diag, m, n, alpha, a.ctypes.data, _ld(a),
+            b.ctypes.data, _ld(b), nb)
+    assert rc == 0, rc
+    return b

@@ -23,7 +23,8 @@
 
 #include "plasma_oracle.h"
 
-enum { ORA_NOTRANS = 111, ORA_TRANS = 112, ORA_LEFT = 141, ORA_RIGHT = 142 };
+enum { ORA_NOTRANS = 111, ORA_TRANS = 112, ORA_LEFT = 141, ORA_RIGHT = 142,
+       ORA_NONUNIT = 131, ORA_UNIT = 132 };
 
 static int ora_backend = ORA_BLAS_BUILTIN;
 static int mkl_layer_refused = 0;
@@ -303,6 +304,43 @@ void ora_tile2lapack(int uplo, int m, int n, const double *tiles, double *pA,
  * Precision conversion (reference core_zlag2c.c:48-79 -> LAPACK dlag2s:
  * info = 1 when an entry exceeds the FP32 range; core_clag2z.c).
  * --------------------------------------------------------------------- */
+/* Tile BLAS-3 on LAPACK arrays (reference compute/zgemm.c, zsyrk.c,
+ * ztrsm.c with pzgemm / pzsyrk / pztrsm). */
+int ora_dgemm(int ta, int tb, int m, int n, int k, double alpha,
+              const double *A, int lda, const double *B, int ldb, double beta,
+              double *C, int ldc, int nb)
+{
+    return d_blas3(0, ta, tb, 0, 0, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                   ldc, nb);
+}
+int ora_dsyrk(int uplo, int trans, int n, int k, double alpha,
+              const double *A, int lda, double beta, double *C, int ldc,
+              int nb)
+{
+    return d_blas3(1, uplo, trans, 0, 0, n, n, k, alpha, A, lda, NULL, 1,
+                   beta, C, ldc, nb);
+}
+int ora_dtrsm(int side, int uplo, int trans, int diag, int m, int n,
+              double alpha, const double *A, int lda, double *B, int ldb,
+              int nb)
+{
+    return d_blas3(2, side, uplo, trans, diag, m, n, 0, alpha, A, lda, NULL,
+                   1, 0, B, ldb, nb);
+}
+int ora_sgemm(int ta, int tb, int m, int n, int k, float alpha,
+              const float *A, int lda, const float *B, int ldb, float beta,
+              float *C, int ldc, int nb)
+{
+    return s_blas3(0, ta, tb, 0, 0, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                   ldc, nb);
+}
+int ora_strsm(int side, int uplo, int trans, int diag, int m, int n,
+              float alpha, const float *A, int lda, float *B, int ldb, int nb)
+{
+    return s_blas3(2, side, uplo, trans, diag, m, n, 0, alpha, A, lda, NULL,
+                   1, 0, B, ldb, nb);
+}
+
 int ora_dlag2s(int m, int n, const double *A, int lda, float *As, int ldas)
 {
     double rmax = (double)FLT_MAX;

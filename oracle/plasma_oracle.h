@@ -73,6 +73,24 @@ int ora_dsposv(int uplo, int n, int nrhs, const double *A, int lda,
 int ora_dpotrf_adjoint(int uplo, int n, const double *L, int ldl,
                        double *Lbar, int ldlb);
 
+/* plasma_dgemm / dsyrk / dtrsm (+ s) with tile size nb (reference
+ * compute/zgemm.c:105, zsyrk.c:89, ztrsm.c:104); enums as the reference
+ * (111 NoTrans, 112 Trans; 141 Left, 142 Right; 131 NonUnit, 132 Unit). */
+int ora_dgemm(int ta, int tb, int m, int n, int k, double alpha,
+              const double *A, int lda, const double *B, int ldb, double beta,
+              double *C, int ldc, int nb);
+int ora_dsyrk(int uplo, int trans, int n, int k, double alpha,
+              const double *A, int lda, double beta, double *C, int ldc,
+              int nb);
+int ora_dtrsm(int side, int uplo, int trans, int diag, int m, int n,
+              double alpha, const double *A, int lda, double *B, int ldb,
+              int nb);
+int ora_sgemm(int ta, int tb, int m, int n, int k, float alpha,
+              const float *A, int lda, const float *B, int ldb, float beta,
+              float *C, int ldc, int nb);
+int ora_strsm(int side, int uplo, int trans, int diag, int m, int n,
+              float alpha, const float *A, int lda, float *B, int ldb, int nb);
+
 /* plasma_dlag2s / plasma_slag2d on LAPACK arrays. */
 int ora_dlag2s(int m, int n, const double *A, int lda, float *As, int ldas);
 void ora_slag2d(int m, int n, const float *As, int ldas, double *A, int lda);

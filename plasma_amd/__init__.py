@@ -128,6 +128,23 @@ for _p, _t in (("d", "double"), ("s", "float")):
         "plasma_omp_%samax" % _p: (None, [c_int, _D, c_void_p, c_void_p,
                                           _PS, _PR]),
     })
+    _S = c_double if _p == "d" else ctypes.c_float
+    PROTOTYPES.update({
+        "plasma_%sgemm" % _p: (c_int, [c_int, c_int, c_int, c_int, c_int, _S,
+                                       c_void_p, c_int, c_void_p, c_int, _S,
+                                       c_void_p, c_int]),
+        "plasma_omp_%sgemm" % _p: (None, [c_int, c_int, _S, _D, _D, _S, _D,
+                                          _PS, _PR]),
+        "plasma_%ssyrk" % _p: (c_int, [c_int, c_int, c_int, c_int, _S,
+                                       c_void_p, c_int, _S, c_void_p, c_int]),
+        "plasma_omp_%ssyrk" % _p: (None, [c_int, c_int, _S, _D, _S, _D, _PS,
+                                          _PR]),
+        "plasma_%strsm" % _p: (c_int, [c_int, c_int, c_int, c_int, c_int,
+                                       c_int, _S, c_void_p, c_int, c_void_p,
+                                       c_int]),
+        "plasma_omp_%strsm" % _p: (None, [c_int, c_int, c_int, c_int, _S, _D,
+                                          _D, _PS, _PR]),
+    })
 PROTOTYPES["plasma_norm_work_doubles"] = (c_size_t, [_D])
 PROTOTYPES.update({
     "plasma_dsposv": (c_int, [c_int, c_int, c_int, c_void_p, c_int, c_void_p,

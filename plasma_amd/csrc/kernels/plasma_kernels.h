@@ -64,6 +64,25 @@ int plasma_kern_trsm_panel_blocked(plasma_enum_t uplo, plasma_desc_t A, int k,
                                    int pcols, int pcol, const int *status,
                                    hipStream_t s);
 
+/* Public BLAS-3 (kernels/blas3.hip).  C = alpha op(A) op(B) + beta C on
+ * general tile descriptors (views allowed), transa / transb PlasmaNoTrans or
+ * (Conj)Trans; tri = PlasmaLower / PlasmaUpper restricts the update to that
+ * triangle of C (syrk), PlasmaGeneral updates all of C.  alpha == 0 reads no
+ * operand, beta == 0 reads no C. */
+int plasma_kern_gemm(int transa, int transb, double alpha, plasma_desc_t A,
+                     plasma_desc_t B, double beta, plasma_desc_t C,
+                     plasma_enum_t tri, const int *status, hipStream_t s);
+/* Diagonal tiles of a triangular A (uplo, diag) as lower-equivalent lower
+ * triangles (unit diagonal for PlasmaUnit, zeros above) into tiles (0, k) of
+ * the one-tile-row descriptor D (D.m = A.mb, D.n = A.n). */
+int plasma_kern_trsm_diag_prep(plasma_enum_t uplo, plasma_enum_t diag,
+                               plasma_desc_t A, plasma_desc_t D,
+                               const int *status, hipStream_t s);
+/* W_k = D(0,k)^{-1} for the lower triangular tiles of D, at
+ * inv + k * mb * mb (leading dimension mb). */
+int plasma_kern_trtri_tiles(plasma_desc_t D, void *inv, const int *status,
+                            hipStream_t s);
+
 /* W_kk = L(k,k)^{-1} for every diagonal tile of an already factored A, one
  * workgroup per tile (used by potrs without a preceding factorization). */
 int plasma_kern_trtri_diag(plasma_enum_t uplo, plasma_desc_t A, void *inv,

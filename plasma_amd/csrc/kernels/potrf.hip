@@ -1033,6 +1033,54 @@ int launch_vbatched(int upper, int batch, const int *n, void *A,
     return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
 }
 
+
+// ---------------------------------------------------------------------------
+// Inverses of the lower triangular tiles (0, k) of a one-tile-row descriptor
+// D (the diagonal tiles of a triangular matrix, as plasma_kern_trsm_diag_prep
+// leaves them), one workgroup per tile: W_k = D(0,k)^{-1}, lower, at
+// inv + k mb^2 with leading dimension mb.  factor_tile's invert-only form.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void trtri_tiles_kernel(
+    plasma_desc_t D, T *inv, const int *status)
+{
+    if (block_should_stop(status)) return;
+    const int k = blockIdx.x;
+    View<T> L;
+    L.p = (T *)plasma_tile_addr(D, 0, k);
+    L.rs = 1;
+    L.cs = plasma_tile_mmain(D, 0);
+    View<T> Wv;
+    Wv.p = inv + (size_t)k * D.mb * D.mb;
+    Wv.rs = 1;
+    Wv.cs = D.mb;
+    extern __shared__ unsigned char smem_raw[];
+    factor_tile<T, NTHREADS>(L, plasma_tile_nview(D, k), Wv, 0,
+                             reinterpret_cast<T *>(smem_raw));
+}
+
+template <typename T>
+int launch_trtri_tiles(plasma_desc_t D, void *inv, const int *status,
+                       hipStream_t s)
+{
+    if (D.nt <= 0) return 0;
+    if (D.mb != D.nb || D.mb > 512) return PlasmaErrorNotSupported;
+    const size_t lds = (size_t)max(D.mb, 8 * JB) * (JB + 1) * sizeof(T);
+    if (lds > 65536) {
+        static int attr_set[2] = {0, 0};
+        int &done = attr_set[sizeof(T) == 8];
+        if (!done) {
+            hipFuncSetAttribute((const void *)trtri_tiles_kernel<T>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds);
+            done = 1;
+        }
+    }
+    hipLaunchKernelGGL(trtri_tiles_kernel<T>, dim3(D.nt), dim3(NTHREADS), lds,
+                       s, D, (T *)inv, status);
+    return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
+}
+
 }  // namespace
 
 extern "C" int plasma_kern_potrf_diag(plasma_enum_t uplo, plasma_desc_t A,
@@ -1156,5 +1204,15 @@ extern "C" int plasma_kern_vbatched_posv(plasma_enum_t precision, int upper,
     if (precision == PlasmaRealFloat)
         return launch_vbatched<float>(upper, batch, n, A, aoff, W, nrhs, B,
                                       boff, info, order, ticket, grid, s);
+    return PlasmaErrorNotSupported;
+}
+
+extern "C" int plasma_kern_trtri_tiles(plasma_desc_t D, void *inv,
+                                       const int *status, hipStream_t s)
+{
+    if (D.precision == PlasmaRealDouble)
+        return launch_trtri_tiles<double>(D, inv, status, s);
+    if (D.precision == PlasmaRealFloat)
+        return launch_trtri_tiles<float>(D, inv, status, s);
     return PlasmaErrorNotSupported;
 }

@@ -320,9 +320,9 @@ class Bench:
         rows, eager = [], []
         for line in open(path):
             f = line.split()
-            if len(f) in (6, 7) and f[0] == "potrf" and f[1] == prec:
+            if 6 <= len(f) <= 8 and f[0] == "potrf" and f[1] == prec:
                 r = tuple(int(x) for x in f[2:6])
-                (eager if f[6:] == ["eager"] else rows).append(r)
+                (eager if f[6:7] == ["eager"] else rows).append(r)
         rows = sorted(rows or eager)
         if not rows:
             raise SystemExit("no %s rows in %s" % (prec, path))

@@ -67,10 +67,11 @@ static int tile_aligned(plasma_desc_t A)
     return A.i % A.mb == 0 && A.j % A.nb == 0;
 }
 
-/* norm of A (uplo PlasmaGeneral: lange) into the caller's value */
+/* norm of A (uplo PlasmaGeneral: lange) into the caller's value, a double
+ * (vprec PlasmaRealDouble) or a float (PlasmaRealFloat) */
 static void omp_norm(plasma_enum_t norm, plasma_enum_t uplo, plasma_desc_t A,
-                     double *work, void *value, plasma_sequence_t *sequence,
-                     plasma_request_t *request)
+                     double *work, void *value, plasma_enum_t vprec,
+                     plasma_sequence_t *sequence, plasma_request_t *request)
 {
     plasma_context_t *ctx = plasma_context_self();
     if (ctx == NULL) {
@@ -151,7 +152,7 @@ static void omp_norm(plasma_enum_t norm, plasma_enum_t uplo, plasma_desc_t A,
     if (err)
         plasma_request_fail(sequence, request, err);
     else
-        deliver(ctx, value, dval, 1, A.precision, sequence, request);
+        deliver(ctx, value, dval, 1, vprec, sequence, request);
     if (own != NULL)
         plasma_dev_free(own);
     else
@@ -290,7 +291,8 @@ static double sync_norm(plasma_enum_t precision, plasma_enum_t norm,
     plasma_pxlapack2desc(sym ? uplo : PlasmaGeneral, pA, lda, A, &sequence,
                          &request);
     /* computed in double for both precisions, delivered to the host */
-    omp_norm(norm, uplo, A, NULL, &value, &sequence, &request);
+    omp_norm(norm, uplo, A, NULL, &value, PlasmaRealDouble, &sequence,
+             &request);
     int status = plasma_sequence_sync(&sequence);
     plasma_desc_destroy(&A);
     return status == PlasmaSuccess ? value : (double)status;
@@ -391,7 +393,7 @@ void plasma_omp_dlansy(plasma_enum_t norm, plasma_enum_t uplo,
         plasma_request_fail(sequence, request, PlasmaErrorIllegalValue);
         return;
     }
-    omp_norm(norm, uplo, A, work, value, sequence, request);
+    omp_norm(norm, uplo, A, work, value, PlasmaRealDouble, sequence, request);
 }
 
 void plasma_omp_slansy(plasma_enum_t norm, plasma_enum_t uplo,
@@ -403,21 +405,23 @@ void plasma_omp_slansy(plasma_enum_t norm, plasma_enum_t uplo,
         plasma_request_fail(sequence, request, PlasmaErrorIllegalValue);
         return;
     }
-    omp_norm(norm, uplo, A, work, value, sequence, request);
+    omp_norm(norm, uplo, A, work, value, PlasmaRealFloat, sequence, request);
 }
 
 void plasma_omp_dlange(plasma_enum_t norm, plasma_desc_t A, double *work,
                        double *value, plasma_sequence_t *sequence,
                        plasma_request_t *request)
 {
-    omp_norm(norm, PlasmaGeneral, A, work, value, sequence, request);
+    omp_norm(norm, PlasmaGeneral, A, work, value, PlasmaRealDouble, sequence,
+             request);
 }
 
 void plasma_omp_slange(plasma_enum_t norm, plasma_desc_t A, double *work,
                        float *value, plasma_sequence_t *sequence,
                        plasma_request_t *request)
 {
-    omp_norm(norm, PlasmaGeneral, A, work, value, sequence, request);
+    omp_norm(norm, PlasmaGeneral, A, work, value, PlasmaRealFloat, sequence,
+             request);
 }
 
 void plasma_omp_damax(plasma_enum_t colrow, plasma_desc_t A, double *work,

@@ -492,6 +492,21 @@ void plasma_graph_release_all(plasma_context_t *ctx)
  * which is loaded from the sequence's word before a replay and stored back
  * after it, so one graph serves every sequence.
  */
+/* Same descriptor, field by field: plasma_desc_t has padding after its
+ * enums that the init functions never write, so a byte compare of two
+ * descriptors of the same matrix can differ. */
+static int desc_same(const plasma_desc_t *a, const plasma_desc_t *b)
+{
+    return a->type == b->type && a->uplo == b->uplo &&
+           a->precision == b->precision && a->matrix == b->matrix &&
+           a->A21 == b->A21 && a->A12 == b->A12 && a->A22 == b->A22 &&
+           a->mb == b->mb && a->nb == b->nb && a->gm == b->gm &&
+           a->gn == b->gn && a->gmt == b->gmt && a->gnt == b->gnt &&
+           a->i == b->i && a->j == b->j && a->m == b->m && a->n == b->n &&
+           a->mt == b->mt && a->nt == b->nt && a->kl == b->kl &&
+           a->ku == b->ku && a->klt == b->klt && a->kut == b->kut;
+}
+
 static plasma_graph_entry_t *graph_get(plasma_enum_t uplo, plasma_desc_t A,
                                        void *inv, int lookahead,
                                        plasma_sequence_t *sequence,
@@ -503,7 +518,7 @@ static plasma_graph_entry_t *graph_get(plasma_enum_t uplo, plasma_desc_t A,
         plasma_graph_entry_t *c = &ctx->graphs[i];
         if (c->used && c->uplo == (int)uplo && c->inv_arg == inv &&
             c->lookahead == lookahead && c->group == ctx->group &&
-            c->split == use_split(A) && memcmp(&c->A, &A, sizeof(A)) == 0)
+            c->split == use_split(A) && desc_same(&c->A, &A))
             e = c;
         if (!c->used || (victim->used && c->stamp < victim->stamp))
             victim = c;

@@ -156,7 +156,9 @@ def test_dsyrk(plasma, nb_guard, uplo, trans):
     opa = A if trans == NT else A.T
     bound = _gemm_bound(0.75, opa, opa.T, 0.5, C, U.EPS64)
     tri = np.tril if uplo == pa.PlasmaLower else np.triu
-    assert np.all(tri(np.abs(got - ref) <= bound))
+    # the stored triangle within the bound (tri of a difference, not of a
+    # boolean mask, whose other triangle would read False)
+    assert np.all(tri(np.abs(got - ref) - bound) <= 0)
     # the other triangle (strictly) is untouched, bit for bit
     other = np.triu if uplo == pa.PlasmaLower else np.tril
     off = 1 if uplo == pa.PlasmaLower else -1

@@ -188,6 +188,9 @@ def parse():
     ap.add_argument("--config2", type=str, default="8192,16384",
                     help="orders of the plain FP64 dpotrf+dpotrs lines on "
                          "G G^T/n + I (tuned nb; empty skips them)")
+    ap.add_argument("--no-half", action="store_true",
+                    help="skip the half-precision-input GMRES-IR solves "
+                         "(plasma_dhposv) of the kappa sweep")
     ap.add_argument("--no-host", action="store_true",
                     help="skip the host-pointer (PCIe-inclusive) timings")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -748,8 +751,9 @@ def main():
         ms_p, _, res_p = bench.plain(A0, B0, "plain " + tag, profile=False)
         # the half-precision-input factors with GMRES-IR on the same system
         half = {}
-        for fmt, val in (("bf16", bench.pa.PlasmaSplitBf16),
-                         ("fp16", bench.pa.PlasmaSplitFp16)):
+        for fmt, val in () if args.no_half else (
+                ("bf16", bench.pa.PlasmaSplitBf16),
+                ("fp16", bench.pa.PlasmaSplitFp16)):
             bench.pa.set_param(bench.pa.PlasmaHalfInput, val)
             half[fmt] = bench.mixed(A0, B0, "%s gmres-ir %s" % (fmt, tag),
                                     profile=False, routine="plasma_dhposv")

@@ -36,6 +36,28 @@ int plasma_dpotrf_vbatched(plasma_enum_t uplo, int batch, const int *n,
 int plasma_spotrf_vbatched(plasma_enum_t uplo, int batch, const int *n,
                            float *A, const size_t *aoff, int *info);
 
+/*
+ * plasma_dposv_vbatched without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory
+ * (code under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results, info values and error codes as
+ * plasma_dposv_vbatched; in addition batch > PLASMA_VBATCH_NOALLOC_MAX
+ * returns -2.  Its scratch (the persistent grid's diagonal-block inverses,
+ * the order of the systems, the ticket, and staged copies of host n / aoff
+ * / boff / info) is one fixed device array per GPU of about 36 MB, reserved
+ * when the library loads and kept until the process ends, so it calls no
+ * hipMalloc, hipFree or allocating library routine.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (from several
+ * host threads) race; calls in sequence are safe.  The call returns when
+ * the batch is done.
+ */
+#define PLASMA_VBATCH_NOALLOC_MAX  65536
+#define PLASMA_VBATCH_NOALLOC_GRID 256
+int plasma_dposv_vbatched_noalloc(plasma_enum_t uplo, int batch,
+                                  const int *n, double *A,
+                                  const size_t *aoff, int nrhs, double *B,
+                                  const size_t *boff, int *info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -165,6 +165,9 @@ PROTOTYPES.update({
     "plasma_sposv_vbatched": (c_int, [c_int, c_int, c_void_p, c_void_p,
                                       c_void_p, c_int, c_void_p, c_void_p,
                                       c_void_p]),
+    "plasma_dposv_vbatched_noalloc": (c_int, [c_int, c_int, c_void_p,
+                                              c_void_p, c_void_p, c_int,
+                                              c_void_p, c_void_p, c_void_p]),
     "plasma_dpotrf_vbatched": (c_int, [c_int, c_int, c_void_p, c_void_p,
                                        c_void_p, c_void_p]),
     "plasma_spotrf_vbatched": (c_int, [c_int, c_int, c_void_p, c_void_p,

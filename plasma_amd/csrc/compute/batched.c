@@ -54,15 +54,13 @@ static int stage_in(const void *arr, size_t bytes, void **dev_out,
                ? PlasmaErrorInternal : 0;
 }
 
-static int vbatched(plasma_enum_t precision, plasma_enum_t uplo, int batch,
-                    const int *n, void *A, const size_t *aoff, int nrhs,
-                    void *B, const size_t *boff, int *info)
+/* Argument checks of the batched drivers, in the order of their codes:
+ * returns -i for an illegal argument i, 1 for the quick return (batch 0),
+ * 0 to proceed. */
+static int vbatched_args(plasma_enum_t uplo, int batch, const int *n,
+                         const void *A, const size_t *aoff, int nrhs,
+                         const void *B, const size_t *boff, const int *info)
 {
-    plasma_context_t *ctx = plasma_context_self();
-    if (ctx == NULL) {
-        plasma_error("PLASMA not initialized");
-        return PlasmaErrorNotInitialized;
-    }
     if (uplo != PlasmaUpper && uplo != PlasmaLower) {
         plasma_error("illegal value of uplo");
         return -1;
@@ -72,7 +70,7 @@ static int vbatched(plasma_enum_t precision, plasma_enum_t uplo, int batch,
         return -2;
     }
     if (batch == 0)
-        return PlasmaSuccess;
+        return 1;
     if (n == NULL) {
         plasma_error("NULL n");
         return -3;
@@ -101,6 +99,35 @@ static int vbatched(plasma_enum_t precision, plasma_enum_t uplo, int batch,
         plasma_error("NULL info");
         return -9;
     }
+    return 0;
+}
+
+/* Workgroups of the persistent grid: as many as the LDS lets the CUs hold,
+ * at most one per system. */
+static int vbatched_grid(plasma_context_t *ctx, plasma_enum_t precision,
+                         int batch)
+{
+    const size_t lds = plasma_kern_vbatched_lds(precision);
+    int per_cu = (int)((160 * 1024) / lds);
+    if (per_cu < 1)
+        per_cu = 1;
+    int grid = ctx->num_cus * per_cu;
+    return grid > batch ? batch : grid;
+}
+
+static int vbatched(plasma_enum_t precision, plasma_enum_t uplo, int batch,
+                    const int *n, void *A, const size_t *aoff, int nrhs,
+                    void *B, const size_t *boff, int *info)
+{
+    plasma_context_t *ctx = plasma_context_self();
+    if (ctx == NULL) {
+        plasma_error("PLASMA not initialized");
+        return PlasmaErrorNotInitialized;
+    }
+    const int args = vbatched_args(uplo, batch, n, A, aoff, nrhs, B, boff,
+                                   info);
+    if (args != 0)
+        return args < 0 ? args : PlasmaSuccess;
 
     /* host view of n: the argument check and the order */
     const size_t nbytes = (size_t)batch * sizeof(int);
@@ -141,15 +168,8 @@ static int vbatched(plasma_enum_t precision, plasma_enum_t uplo, int batch,
     for (int i = 0; i < batch; i++)
         hord[i] = ord[i].i;
 
-    /* one resident grid: as many workgroups as the LDS lets the CUs hold,
-     * each with its own slab of W */
-    const size_t lds = plasma_kern_vbatched_lds(precision);
-    int per_cu = (int)((160 * 1024) / lds);
-    if (per_cu < 1)
-        per_cu = 1;
-    int grid = ctx->num_cus * per_cu;
-    if (grid > batch)
-        grid = batch;
+    /* one resident grid, each workgroup with its own slab of W */
+    const int grid = vbatched_grid(ctx, precision, batch);
     W = plasma_dev_alloc((size_t)grid * plasma_kern_vbatched_wslab(precision));
     dord = plasma_dev_alloc(nbytes);
     dticket = plasma_dev_alloc(sizeof(int));
@@ -243,4 +263,101 @@ int plasma_spotrf_vbatched(plasma_enum_t uplo, int batch, const int *n,
 {
     return vbatched(PlasmaRealFloat, uplo, batch, n, A, aoff, 0, NULL, NULL,
                     info);
+}
+
+/* The allocation-free variant: every device buffer beyond the caller's
+ * inputs and outputs is a region of the per-device scratch array of
+ * kernels/noalloc.hip (its address looked up once per device), and no host
+ * buffer is allocated either.  Host metadata arrays are copied into the
+ * scratch; the systems are ordered on the device. */
+int plasma_dposv_vbatched_noalloc(plasma_enum_t uplo, int batch,
+                                  const int *n, double *A,
+                                  const size_t *aoff, int nrhs, double *B,
+                                  const size_t *boff, int *info)
+{
+    plasma_context_t *ctx = plasma_context_self();
+    if (ctx == NULL) {
+        plasma_error("PLASMA not initialized");
+        return PlasmaErrorNotInitialized;
+    }
+    const int args = vbatched_args(uplo, batch, n, A, aoff, nrhs, B, boff,
+                                   info);
+    if (args != 0)
+        return args < 0 ? args : PlasmaSuccess;
+    if (batch > PLASMA_VBATCH_NOALLOC_MAX) {
+        plasma_error("batch above PLASMA_VBATCH_NOALLOC_MAX, the number of "
+                     "systems the allocation-free scratch holds");
+        return -2;
+    }
+    plasma_noalloc_t r;
+    if (plasma_noalloc_setup(ctx->device) != 0 ||
+        plasma_noalloc_regions(ctx->device, &r) != 0 ||
+        plasma_kern_vbatched_wslab(PlasmaRealDouble) > r.wslab_bytes) {
+        plasma_error("allocation-free scratch unavailable");
+        return PlasmaErrorInternal;
+    }
+    hipStream_t s = ctx->stream;
+    const size_t nbytes = (size_t)batch * sizeof(int);
+    const size_t obytes = (size_t)batch * sizeof(size_t);
+    const int host_n = !plasma_is_device_ptr(n);
+    const int *dn = n;
+    if (host_n) {
+        /* checked here, like plasma_dposv_vbatched, before any launch */
+        for (int i = 0; i < batch; i++)
+            if (n[i] < 0 || n[i] > VBATCH_NMAX) {
+                plasma_error("illegal order of a system");
+                return -3;
+            }
+        if (plasma_hip_check(hipMemcpyAsync(r.n, n, nbytes,
+                                            hipMemcpyHostToDevice, s),
+                             "hipMemcpyAsync(n)"))
+            return PlasmaErrorInternal;
+        dn = r.n;
+    }
+    const size_t *daoff = aoff, *dboff = boff;
+    if (!plasma_is_device_ptr(aoff)) {
+        if (plasma_hip_check(hipMemcpyAsync(r.aoff, aoff, obytes,
+                                            hipMemcpyHostToDevice, s),
+                             "hipMemcpyAsync(aoff)"))
+            return PlasmaErrorInternal;
+        daoff = r.aoff;
+    }
+    if (nrhs > 0 && !plasma_is_device_ptr(boff)) {
+        if (plasma_hip_check(hipMemcpyAsync(r.boff, boff, obytes,
+                                            hipMemcpyHostToDevice, s),
+                             "hipMemcpyAsync(boff)"))
+            return PlasmaErrorInternal;
+        dboff = r.boff;
+    }
+    const int host_info = !plasma_is_device_ptr(info);
+    int *dinfo = host_info ? r.info : info;
+    if (plasma_kern_vbatched_order(dn, batch, r.order, r.ctrl, s) != 0)
+        return PlasmaErrorInternal;
+    if (!host_n) {
+        /* orders in device memory: checked by the order kernel */
+        int ctrl[2] = {0, 0};
+        if (plasma_hip_check(hipMemcpyAsync(ctrl, r.ctrl, sizeof(ctrl),
+                                            hipMemcpyDeviceToHost, s),
+                             "hipMemcpyAsync(flag)") ||
+            plasma_hip_check(hipStreamSynchronize(s), "hipStreamSynchronize"))
+            return PlasmaErrorInternal;
+        if (ctrl[1] != 0) {
+            plasma_error("illegal order of a system");
+            return -3;
+        }
+    }
+    int grid = vbatched_grid(ctx, PlasmaRealDouble, batch);
+    if (grid > r.grid_max)
+        grid = r.grid_max;
+    if (plasma_kern_vbatched_posv(PlasmaRealDouble, uplo == PlasmaUpper,
+                                  batch, dn, A, daoff, r.W, nrhs, B, dboff,
+                                  dinfo, r.order, r.ctrl, grid, s) != 0)
+        return PlasmaErrorInternal;
+    if (host_info &&
+        plasma_hip_check(hipMemcpyAsync(info, r.info, nbytes,
+                                        hipMemcpyDeviceToHost, s),
+                         "hipMemcpyAsync(info)"))
+        return PlasmaErrorInternal;
+    return plasma_hip_check(hipStreamSynchronize(s), "hipStreamSynchronize")
+               ? PlasmaErrorInternal : PlasmaSuccess;
 }

@@ -120,6 +120,11 @@ int plasma_init(void)
         plasma_hip_check(hipStreamSynchronize(ctx->stream), "init sync"))
         return PlasmaErrorEnvironment;
 
+    /* the address of this device's allocation-free scratch
+     * (kernels/noalloc.hip), looked up once with the device current */
+    if (plasma_noalloc_setup(dev) != 0)
+        plasma_warning("allocation-free batched scratch unavailable");
+
     plasma_initialized_g = 1;
     plasma_tuning_init(ctx);
     return PlasmaSuccess;

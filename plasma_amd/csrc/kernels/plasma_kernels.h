@@ -18,6 +18,7 @@
 
 #include <hip/hip_runtime_api.h>
 #include "plasma_descriptor.h"
+#include "plasma_batched.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -171,6 +172,29 @@ int plasma_kern_vbatched_posv(plasma_enum_t precision, int upper, int batch,
                               void *W, int nrhs, void *B, const size_t *boff,
                               int *info, const int *order, int *ticket,
                               int grid, hipStream_t s);
+
+/*
+ * Scratch of the allocation-free batched solve (kernels/noalloc.hip): one
+ * file-scope device array per device, its address looked up at
+ * plasma_noalloc_setup (called by plasma_init with the device current);
+ * plasma_noalloc_regions hands out its parts.  plasma_kern_vbatched_order
+ * writes the systems largest order first into `order`, zeroes the ticket
+ * ctrl[0] and sets ctrl[1] when an order is outside [0, 512]; n, order and
+ * ctrl are device memory, batch <= PLASMA_VBATCH_NOALLOC_MAX.
+ */
+typedef struct {
+    void *W;                 /* grid_max slabs of wslab_bytes */
+    size_t wslab_bytes;
+    int grid_max;
+    int *ctrl;               /* [0] ticket, [1] invalid-order flag */
+    int *n, *order, *info;   /* PLASMA_VBATCH_NOALLOC_MAX entries each */
+    size_t *aoff, *boff;
+    size_t bytes;            /* the whole array */
+} plasma_noalloc_t;
+int plasma_noalloc_setup(int device);
+int plasma_noalloc_regions(int device, plasma_noalloc_t *r);
+int plasma_kern_vbatched_order(const int *n, int batch, int *order,
+                               int *ctrl, hipStream_t s);
 
 /*
  * Adjoint kernels (no reference counterpart).  ltl_lower: Q = the symmetric

@@ -103,10 +103,8 @@ static void precond(const dh_t *d, plasma_desc_t Y, plasma_desc_t Z,
 {
     if (issue(plasma_kern_lag2(Y, d->Xs, d->st, d->s), sequence, request))
         return;
-    if (d->mpan != NULL)
-        plasma_pxpotrs_m(d->As, d->Xs, d->inv_s, d->mpan, sequence, request);
-    else
-        plasma_pxpotrs(d->uplo, d->As, d->Xs, d->inv_s, sequence, request);
+    plasma_rsolve(d->uplo, d->As, d->Xs, d->inv_s, d->mpan, sequence,
+                  request);
     if (sequence->status == PlasmaSuccess)
         issue(plasma_kern_lag2(d->Xs, Z, d->st, d->s), sequence, request);
 }
@@ -435,10 +433,8 @@ int plasma_dhposv(plasma_enum_t uplo, int n, int nrhs, double *pA, int lda,
         fallback = 1;
     }
     else {
-        d.mpan = plasma_dev_alloc(plasma_mpanel_bytes(As));
-        if (d.mpan != NULL)
-            plasma_pxscale_panels(uplo, As, d.inv_s, d.mpan, &sequence,
-                                  &request);
+        d.mpan = plasma_rsolve_prepare(uplo, As, d.inv_s, &sequence,
+                                       &request);
         /* X = M^{-1} S B, then refinement steps */
         precond(&d, B, X, &sequence, &request);
         int conv = 0;

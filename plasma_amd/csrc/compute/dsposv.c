@@ -58,16 +58,13 @@ static int read_conv(plasma_context_t *ctx, const int *dflag, const int *st,
     return flag ? 1 : 0;
 }
 
-/* FP32 solve with the factor in As: through the scaled panels when they
- * could be allocated, else the plain sweeps */
+/* FP32 solve with the factor in As, in its repeated-solve form when that
+ * could be allocated (plasma_rsolve), else by the plain sweeps */
 static void solve_s(plasma_enum_t uplo, plasma_desc_t As, plasma_desc_t Xs,
                     const void *inv_s, const void *mpan,
                     plasma_sequence_t *sequence, plasma_request_t *request)
 {
-    if (mpan != NULL)
-        plasma_pxpotrs_m(As, Xs, inv_s, mpan, sequence, request);
-    else
-        plasma_pxpotrs(uplo, As, Xs, inv_s, sequence, request);
+    plasma_rsolve(uplo, As, Xs, inv_s, mpan, sequence, request);
 }
 
 /* R = B - A X, the column max-norms of X and R and the convergence flag
@@ -208,11 +205,9 @@ void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
             goto done;
     }
     plasma_pxpotrf(uplo, As, inv_s, sequence, request);
-    /* the refinement solves with this factor up to 31 times: scale the
-     * panels once so that every sweep step is one launch */
-    mpan = plasma_dev_alloc(plasma_mpanel_bytes(As));
-    if (mpan != NULL)
-        plasma_pxscale_panels(uplo, As, inv_s, mpan, sequence, request);
+    /* the refinement solves with this factor up to 31 times: prepare its
+     * repeated-solve form once (super-block inverses or scaled panels) */
+    mpan = plasma_rsolve_prepare(uplo, As, inv_s, sequence, request);
     solve_s(uplo, As, Xs, inv_s, mpan, sequence, request);
     if (lag2_step(Xs, X, st, s, sequence, request))
         goto done;

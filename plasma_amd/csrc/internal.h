@@ -219,6 +219,26 @@ void plasma_pxscale_panels(plasma_enum_t uplo, plasma_desc_t A,
 void plasma_pxpotrs_m(plasma_desc_t A, plasma_desc_t B, const void *inv,
                       const void *M, plasma_sequence_t *sequence,
                       plasma_request_t *request);
+/* Repeated solves by super-blocks of plasma_sweep_tiles(A) tiles (0: the
+ * path does not apply): V = the diagonal super-blocks' inverses
+ * (plasma_sbinv_bytes(A) bytes), computed once by pxsbinv; then pxpotrs_sb
+ * solves with two launches per super-block and sweep. */
+int plasma_sweep_tiles(plasma_desc_t A);
+size_t plasma_sbinv_bytes(plasma_desc_t A);
+void plasma_pxsbinv(plasma_enum_t uplo, plasma_desc_t A, const void *inv,
+                    void *V, plasma_sequence_t *sequence,
+                    plasma_request_t *request);
+void plasma_pxpotrs_sb(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
+                       const void *V, plasma_sequence_t *sequence,
+                       plasma_request_t *request);
+/* Either form for a factor solved many times: rsolve_prepare computes it
+ * into one pool block (NULL: none, the plain sweeps run), rsolve solves. */
+void *plasma_rsolve_prepare(plasma_enum_t uplo, plasma_desc_t A,
+                            const void *inv, plasma_sequence_t *sequence,
+                            plasma_request_t *request);
+void plasma_rsolve(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
+                   const void *inv, const void *form,
+                   plasma_sequence_t *sequence, plasma_request_t *request);
 
 /* layout translation between a LAPACK array (host or device) and tiles */
 void plasma_pxlapack2desc(plasma_enum_t region, const void *pA, int lda,

@@ -174,6 +174,23 @@ int plasma_kern_vbatched_posv(plasma_enum_t precision, int upper, int batch,
                               int grid, hipStream_t s);
 
 /*
+ * Super-block sweeps (kernels/ssweep.hip), nb a multiple of 64, S tiles per
+ * super-block.  plasma_kern_sbinv: V (ceil(mt / S) slabs of (S nb)^2
+ * elements, ld S nb) = the inverses of the diagonal super-blocks of the
+ * lower-equivalent factor, from the tile inverses inv; Tscr holds
+ * plasma_sbinv_scratch_elems(A, S) elements.  plasma_kern_ssweep: step K of
+ * the forward (backward = 0: B <- L^{-1} B) or backward (B <- L^{-T} B)
+ * sweep, two launches; Y holds S nb x (B.nt nb) elements.
+ */
+size_t plasma_sbinv_scratch_elems(plasma_desc_t A, int S);
+int plasma_kern_sbinv(plasma_enum_t uplo, plasma_desc_t A, const void *inv,
+                      void *V, void *Tscr, int S, const int *status,
+                      hipStream_t s);
+int plasma_kern_ssweep(plasma_enum_t uplo, int backward, plasma_desc_t A,
+                       plasma_desc_t B, const void *V, void *Y, int S, int K,
+                       const int *status, hipStream_t s);
+
+/*
  * Scratch of the allocation-free batched solve (kernels/noalloc.hip): one
  * file-scope device array per device, its address looked up at
  * plasma_noalloc_setup (called by plasma_init with the device current);

@@ -66,6 +66,28 @@ def test_dsposv_iterations_and_residual(plasma, nb_guard, uplo, kappa, margin,
     assert U.scaled_residual(A, X, B) < 10 * U.EPS64
 
 
+@pytest.mark.parametrize("uplo", [L, Up])
+@pytest.mark.parametrize("n,nb,nrhs", [(2500, 128, 37), (1900, 64, 3)])
+def test_dsposv_superblock_sweeps(plasma, nb_guard, uplo, n, nb, nrhs):
+    """The refinement solves run by super-blocks of 1024 rows (inverses of
+    the diagonal super-blocks built once per factor): several super-blocks,
+    the last one partial and ending in a short tile, more right-hand sides
+    than one 32-column block.  Iterations, solution and residual against
+    the oracle's tile-by-tile algorithm."""
+    pa.set_param(pa.PlasmaNb, nb)
+    A = _cond_spd(n, 1e4, seed=n)
+    B = U.rhs(n, nrhs, seed=n + 1)
+    X, it, info = _dsposv(uplo, A, B, device=True)
+    assert info == 0
+    ora.set_blas(ora.MKL)
+    Xo, ito, infoo = ora.dsposv(ora.LOWER if uplo == L else ora.UPPER, A, B,
+                                nb)
+    assert infoo == 0 and 0 <= ito <= 30
+    assert abs(it - ito) <= 1, (it, ito)
+    assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) <= 100 * 1e4 * U.EPS64
+    assert U.scaled_residual(A, X, B) < 10 * U.EPS64
+
+
 def test_dsposv_fp64_fallback(plasma, nb_guard):
     """kappa = 1e8: refinement with an FP32 factor stalls; after 30 steps
     iter = -31 and the FP64 factorization solves (zcposv.c:524-538)."""

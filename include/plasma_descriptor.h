@@ -206,6 +206,22 @@ int plasma_desc_triangular_init(plasma_enum_t precision, plasma_enum_t uplo,
                                 void *matrix, int mb, int nb, int lm, int ln,
                                 int i, int j, int m, int n, plasma_desc_t *A);
 
+/* Band descriptors (reference control/descriptor.c:50-66, :178-212,
+ * :322-371): general storage of lm rows (a multiple of mb covering the
+ * band) by ln columns; tile (m, n) of the band matrix is stored at tile
+ * ((kut - 1) + m - n, n).  uplo PlasmaLower keeps kl subdiagonals, Upper
+ * ku superdiagonals, General both. */
+int plasma_desc_general_band_create(plasma_enum_t precision,
+                                    plasma_enum_t uplo, int mb, int nb,
+                                    int lm, int ln, int i, int j, int m,
+                                    int n, int kl, int ku, plasma_desc_t *A);
+int plasma_desc_general_band_init(plasma_enum_t precision,
+                                  plasma_enum_t uplo, void *matrix, int mb,
+                                  int nb, int lm, int ln, int i, int j,
+                                  int m, int n, int kl, int ku,
+                                  plasma_desc_t *A);
+int plasma_desc_general_band_check(plasma_desc_t A);
+
 int plasma_desc_check(plasma_desc_t A);
 int plasma_desc_general_check(plasma_desc_t A);
 

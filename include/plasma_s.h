@@ -119,6 +119,35 @@ void plasma_omp_strsm(plasma_enum_t side, plasma_enum_t uplo,
                       plasma_desc_t A, plasma_desc_t B,
                       plasma_sequence_t *sequence, plasma_request_t *request);
 
+/*
+ * Band Cholesky (reference compute/zpbtrf.c:77, :200; compute/zpbtrs.c:81,
+ * :236; compute/zpbsv.c:92; compute/zpb2desc.c:27; compute/zdesc2pb.c:27).
+ * AB: the kd + 1 diagonals of the stored triangle in LAPACK band layout
+ * (ldab >= kd + 1), host or device memory; on exit of pbtrf / pbsv the
+ * factor in the same layout.  Return codes as the reference (pbtrf: -1
+ * uplo, -2 n, -3 kd, -5 ldab; pbtrs: -1, -2, -4 kd, -5 nrhs, -7 ldab,
+ * -10 ldb; pbsv: -1, -2, -3 kd, -4 nrhs, -6 ldab, -8 ldb; > 0 the order of
+ * the first leading minor that is not positive definite).  Tile forms: a
+ * band descriptor (plasma_desc_general_band_create) of the uplo triangle.
+ */
+int plasma_spbtrf(plasma_enum_t uplo, int n, int kd, float *pAB, int ldab);
+int plasma_spbtrs(plasma_enum_t uplo, int n, int kd, int nrhs, float *pAB,
+                  int ldab, float *pB, int ldb);
+int plasma_spbsv(plasma_enum_t uplo, int n, int kd, int nrhs, float *pAB,
+                 int ldab, float *pB, int ldb);
+void plasma_omp_spbtrf(plasma_enum_t uplo, plasma_desc_t AB,
+                       plasma_sequence_t *sequence, plasma_request_t *request);
+void plasma_omp_spbtrs(plasma_enum_t uplo, plasma_desc_t AB, plasma_desc_t B,
+                       plasma_sequence_t *sequence, plasma_request_t *request);
+void plasma_omp_spbsv(plasma_enum_t uplo, plasma_desc_t AB, plasma_desc_t B,
+                      plasma_sequence_t *sequence, plasma_request_t *request);
+void plasma_omp_spb2desc(float *pA, int lda, plasma_desc_t A,
+                         plasma_sequence_t *sequence,
+                         plasma_request_t *request);
+void plasma_omp_sdesc2pb(plasma_desc_t A, float *pA, int lda,
+                         plasma_sequence_t *sequence,
+                         plasma_request_t *request);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,6 +40,10 @@ def load():
         sig = {
             "ora_set_blas": (i, [i]), "ora_get_blas": (i, []),
             "ora_mkl_refused": (i, []),
+            "ora_dpbtrf": (i, [i, i, i, p, i, i]),
+            "ora_spbtrf": (i, [i, i, i, p, i, i]),
+            "ora_dpbsv": (i, [i, i, i, i, p, i, p, i, i, i]),
+            "ora_spbsv": (i, [i, i, i, i, p, i, p, i, i, i]),
             "ora_dpotri": (i, [i, i, p, i, i]),
             "ora_dpoinv": (i, [i, i, p, i, i]),
             "ora_spotri": (i, [i, i, p, i, i]),
@@ -116,6 +120,28 @@ def posv(uplo, a, b, nb, dtype=np.float64):
     info = fn(uplo, a.shape[0], b.shape[1], a.ctypes.data, _ld(a),
               b.ctypes.data, _ld(b), nb)
     return a, b, info
+
+
+def pbtrf(uplo, ab, kd, nb, dtype=np.float64):
+    """Band Cholesky of the LAPACK band array ab ((ldab >= kd + 1) x n, the
+    uplo triangle's kd + 1 diagonals); returns (factor band array, info),
+    the input is not modified (reference compute/zpbtrf.c)."""
+    ab = _f(ab, dtype).copy(order="F")
+    fn = load().ora_dpbtrf if dtype == np.float64 else load().ora_spbtrf
+    info = fn(uplo, ab.shape[1], kd, ab.ctypes.data, _ld(ab), nb)
+    return ab, info
+
+
+def pbsv(uplo, ab, kd, b, nb, factor=True, dtype=np.float64):
+    """pbsv (factor=True: returns (factor, X, info)) or pbtrs (factor=False:
+    ab holds the factor; returns (ab, X, info)), reference zpbsv.c /
+    zpbtrs.c."""
+    ab = _f(ab, dtype).copy(order="F")
+    b = _f(b, dtype).copy(order="F")
+    fn = load().ora_dpbsv if dtype == np.float64 else load().ora_spbsv
+    info = fn(uplo, ab.shape[1], kd, b.shape[1], ab.ctypes.data, _ld(ab),
+              b.ctypes.data, _ld(b), nb, 1 if factor else 0)
+    return ab, b, info
 
 
 def potri(uplo, f, nb, dtype=np.float64):

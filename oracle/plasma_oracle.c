@@ -210,6 +210,50 @@ static int ora_nview(const ora_desc_t *A, int k)
     return (A->n % A->nb) ? A->n % A->nb : A->nb;
 }
 
+/* ---------------------------------------------------------------------
+ * Band tile storage: bt x nt full nb x nb tiles (ld nb), bt = 1 +
+ * ceil(kd / nb) tile rows per band column (reference zpbtrf.c:104-110,
+ * descriptor.c:199-211); tile (m, n) of the band at slot (kut - 1 + m - n,
+ * n), kut = bt for upper storage, 1 for lower.
+ * --------------------------------------------------------------------- */
+typedef struct {
+    int nb, n, nt, kd, bt, upper;
+    size_t es;
+    void *mat;
+} ora_band_t;
+
+static int ora_band_create(ora_band_t *A, int upper, size_t es, int nb,
+                           int n, int kd)
+{
+    A->nb = nb;
+    A->n = n;
+    A->nt = (n + nb - 1) / nb;
+    A->kd = kd;
+    A->bt = 1 + (kd + nb - 1) / nb;
+    A->upper = upper;
+    A->es = es;
+    A->mat = calloc((size_t)A->bt * A->nt * nb * nb, es);
+    return A->mat == NULL;
+}
+
+static void ora_band_destroy(ora_band_t *A) { free(A->mat); A->mat = NULL; }
+
+static void *ora_band_tile(const ora_band_t *A, int m, int n)
+{
+    const int slot = A->upper ? (A->bt - 1) + m - n : m - n;
+    return (char *)A->mat +
+           ((size_t)slot + (size_t)A->bt * n) * A->nb * A->nb * A->es;
+}
+
+static int ora_band_view(const ora_band_t *A, int k)
+{
+    if (k < A->nt - 1) return A->nb;
+    return (A->n % A->nb) ? A->n % A->nb : A->nb;
+}
+
+static int imin_(int a, int b) { return a < b ? a : b; }
+static int imax_(int a, int b) { return a > b ? a : b; }
+
 /* ------------------------- double precision -------------------------- */
 #define REAL double
 #define FN(name) d_##name
@@ -232,6 +276,24 @@ static int ora_nview(const ora_desc_t *A, int k)
 #undef CBLAS
 #undef MKL
 
+int ora_dpbtrf(int uplo, int n, int kd, double *AB, int ldab, int nb)
+{
+    return d_band(uplo, n, kd, -1, AB, ldab, NULL, 1, nb, 1);
+}
+int ora_spbtrf(int uplo, int n, int kd, float *AB, int ldab, int nb)
+{
+    return s_band(uplo, n, kd, -1, AB, ldab, NULL, 1, nb, 1);
+}
+int ora_dpbsv(int uplo, int n, int kd, int nrhs, double *AB, int ldab,
+              double *B, int ldb, int nb, int factor)
+{
+    return d_band(uplo, n, kd, nrhs, AB, ldab, B, ldb, nb, factor);
+}
+int ora_spbsv(int uplo, int n, int kd, int nrhs, float *AB, int ldab,
+              float *B, int ldb, int nb, int factor)
+{
+    return s_band(uplo, n, kd, nrhs, AB, ldab, B, ldb, nb, factor);
+}
 int ora_dpotrf(int uplo, int n, double *A, int lda, int nb)
 {
     return d_potrf(uplo, n, A, lda, nb);

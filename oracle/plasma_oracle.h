@@ -37,6 +37,16 @@ int ora_mkl_refused(void);
 int ora_dpotri(int uplo, int n, double *A, int lda, int nb);
 int ora_dpoinv(int uplo, int n, double *A, int lda, int nb);
 int ora_spotri(int uplo, int n, float *A, int lda, int nb);
+/* band Cholesky with tile size nb (reference compute/zpbtrf.c, zpbtrs.c,
+ * zpbsv.c): AB in LAPACK band layout (ldab >= kd + 1); ora_xpbsv with
+ * factor = 1 factors and solves, with factor = 0 solves with the factor in
+ * AB (pbtrs) */
+int ora_dpbtrf(int uplo, int n, int kd, double *AB, int ldab, int nb);
+int ora_spbtrf(int uplo, int n, int kd, float *AB, int ldab, int nb);
+int ora_dpbsv(int uplo, int n, int kd, int nrhs, double *AB, int ldab,
+              double *B, int ldb, int nb, int factor);
+int ora_spbsv(int uplo, int n, int kd, int nrhs, float *AB, int ldab,
+              float *B, int ldb, int nb, int factor);
 /* norms (reference compute/zlansy.c, zlange.c, dzamax.c) */
 double ora_dlansy(int norm, int uplo, int n, const double *A, int lda);
 double ora_dlange(int norm, int m, int n, const double *A, int lda);

@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libplasma_amd.so")
 PlasmaRealFloat, PlasmaRealDouble = 2, 3
 PlasmaNoTrans, PlasmaTrans, PlasmaConjTrans = 111, 112, 113
 PlasmaUpper, PlasmaLower, PlasmaGeneral = 121, 122, 123
+PlasmaGeneralBand = 124
 PlasmaNonUnit, PlasmaUnit = 131, 132
 PlasmaLeft, PlasmaRight = 141, 142
 PlasmaInfNorm = 175
@@ -94,6 +95,10 @@ PROTOTYPES = {
                                     + [c_int] * 8 + [_PD]),
     "plasma_desc_check": (c_int, [_D]),
     "plasma_desc_general_check": (c_int, [_D]),
+    "plasma_desc_general_band_create": (c_int, [c_int] * 12 + [_PD]),
+    "plasma_desc_general_band_init": (c_int, [c_int, c_int, c_void_p]
+                                      + [c_int] * 10 + [_PD]),
+    "plasma_desc_general_band_check": (c_int, [_D]),
     "plasma_desc_view": (_D, [_D, c_int, c_int, c_int, c_int]),
     "plasma_desc_storage_bytes": (c_size_t, [_D]),
 }
@@ -129,6 +134,18 @@ for _p, _t in (("d", "double"), ("s", "float")):
                                           _PS, _PR]),
     })
     _S = c_double if _p == "d" else ctypes.c_float
+    PROTOTYPES.update({
+        "plasma_%spbtrf" % _p: (c_int, [c_int, c_int, c_int, c_void_p, c_int]),
+        "plasma_%spbtrs" % _p: (c_int, [c_int, c_int, c_int, c_int, c_void_p,
+                                        c_int, c_void_p, c_int]),
+        "plasma_%spbsv" % _p: (c_int, [c_int, c_int, c_int, c_int, c_void_p,
+                                       c_int, c_void_p, c_int]),
+        "plasma_omp_%spbtrf" % _p: (None, [c_int, _D, _PS, _PR]),
+        "plasma_omp_%spbtrs" % _p: (None, [c_int, _D, _D, _PS, _PR]),
+        "plasma_omp_%spbsv" % _p: (None, [c_int, _D, _D, _PS, _PR]),
+        "plasma_omp_%spb2desc" % _p: (None, [c_void_p, c_int, _D, _PS, _PR]),
+        "plasma_omp_%sdesc2pb" % _p: (None, [_D, c_void_p, c_int, _PS, _PR]),
+    })
     PROTOTYPES.update({
         "plasma_%sgemm" % _p: (c_int, [c_int, c_int, c_int, c_int, c_int, _S,
                                        c_void_p, c_int, c_void_p, c_int, _S,

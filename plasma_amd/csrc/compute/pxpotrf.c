@@ -901,3 +901,106 @@ void plasma_rsolve(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
     else
         plasma_pxpotrs_m(A, B, inv, form, sequence, request);
 }
+
+/*
+ * Band Cholesky (reference compute/pzpbtrf.c:37-70 lower, :72-117 upper).
+ * Column by column on a band descriptor: the diagonal tile k (with W_kk),
+ * its panel (the band's tiles below it, into a panel buffer) and the update
+ * of the band's window (k, k + bt) with the panel copied back, where bt is
+ * the number of tile rows of a band column (diagonal included).  Every
+ * launch takes a copy of the descriptor cut to that window (mt = nt =
+ * min(mt, k + bt)), so the tile loops of the dense kernels never leave the
+ * band: the tiles outside it are not stored.  The factor keeps the band's
+ * zeros exactly (a zero outside the band is only ever multiplied into
+ * another zero's place).
+ */
+static plasma_desc_t band_window(plasma_desc_t A, int t_end)
+{
+    plasma_desc_t W = A;
+    if (t_end < A.mt) {
+        W.mt = W.nt = t_end;
+        W.m = W.n = t_end * A.mb;
+    }
+    return W;
+}
+
+int plasma_band_tiles(plasma_enum_t uplo, plasma_desc_t A)
+{
+    return uplo == PlasmaUpper ? A.kut : A.klt;
+}
+
+void plasma_pxpbtrf(plasma_enum_t uplo, plasma_desc_t A, void *inv,
+                    plasma_sequence_t *sequence, plasma_request_t *request)
+{
+    if (sequence->status != PlasmaSuccess)
+        return;
+    plasma_context_t *ctx = plasma_context_self();
+    int *st = plasma_sequence_dstatus(sequence, request);
+    hipStream_t s0 = ctx->stream;
+    const int bt = plasma_band_tiles(uplo, A);
+    const size_t es = plasma_element_size(A.precision);
+    void *panel = plasma_dev_alloc((size_t)imax(bt - 1, 1) * A.mb * A.nb *
+                                   es);
+    void *own_inv = inv == NULL ? plasma_dev_alloc(plasma_inv_bytes(A))
+                                : NULL;
+    if (panel == NULL || (inv == NULL && own_inv == NULL)) {
+        plasma_dev_free(panel);
+        plasma_dev_free(own_inv);
+        plasma_request_fail(sequence, request, PlasmaErrorOutOfMemory);
+        return;
+    }
+    for (int k = 0; k < A.mt; k++) {
+        const int t_end = imin(A.mt, k + bt);
+        const group_t g = {uplo, band_window(A, t_end),
+                           inv != NULL ? inv : own_inv, st, A.nb,
+                           SPLIT_NONE, 0, NULL};
+        if (factor_group(&g, k, 1, panel, s0, sequence, request) ||
+            issue(k_update(&g, k, 1, k + 1, t_end, panel, 1, 0, s0),
+                  sequence, request))
+            break;
+    }
+    plasma_dev_free(panel);
+    plasma_dev_free(own_inv);
+}
+
+/* B <- A^{-1} B with the band factor (reference compute/pztbsm.c, the two
+ * sweeps of zpbtrs.c:280-294): per tile the diagonal product with W_kk and
+ * the update of the band's block rows only. */
+void plasma_pxpbtrs(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
+                    const void *inv, plasma_sequence_t *sequence,
+                    plasma_request_t *request)
+{
+    if (sequence->status != PlasmaSuccess)
+        return;
+    plasma_context_t *ctx = plasma_context_self();
+    int *st = plasma_sequence_dstatus(sequence, request);
+    hipStream_t s0 = ctx->stream;
+    const int bt = plasma_band_tiles(uplo, A);
+    const size_t es = plasma_element_size(B.precision);
+    void *scratch = plasma_dev_alloc((size_t)A.mb * B.nt * B.nb * es);
+    if (scratch == NULL) {
+        plasma_request_fail(sequence, request, PlasmaErrorOutOfMemory);
+        return;
+    }
+    int err = 0, slot;
+    plasma_prof_begin(s0, PlasmaProfSweep,
+                      4.0 * B.n * (double)A.m * imin(A.m, bt * A.mb), 0,
+                      &slot);
+    for (int k = 0; k < B.mt && !err; k++)
+        err = issue(plasma_kern_trsm_diag_left(uplo, 0, A, B, k, inv,
+                                               scratch, st, s0),
+                    sequence, request) ||
+              issue(plasma_kern_sweep_update_range(uplo, 0, A, B, k, 0,
+                                                   k + bt, scratch, st, s0),
+                    sequence, request);
+    for (int k = B.mt - 1; k >= 0 && !err; k--)
+        err = issue(plasma_kern_trsm_diag_left(uplo, 1, A, B, k, inv,
+                                               scratch, st, s0),
+                    sequence, request) ||
+              issue(plasma_kern_sweep_update_range(uplo, 1, A, B, k,
+                                                   k - bt + 1, k, scratch,
+                                                   st, s0),
+                    sequence, request);
+    plasma_prof_end(s0, slot);
+    plasma_dev_free(scratch);
+}

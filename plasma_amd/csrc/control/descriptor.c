@@ -181,11 +181,115 @@ int plasma_desc_general_check(plasma_desc_t A)
     return PlasmaSuccess;
 }
 
+/* The band parameters as the reference derives them: the band's tile rows
+ * below (klt) and above (kut) the diagonal, 1 for the diagonal itself. */
+int plasma_desc_general_band_init(plasma_enum_t precision,
+                                  plasma_enum_t uplo, void *matrix, int mb,
+                                  int nb, int lm, int ln, int i, int j,
+                                  int m, int n, int kl, int ku,
+                                  plasma_desc_t *A)
+{
+    int retval = plasma_desc_general_init(precision, matrix, mb, nb, lm, ln,
+                                          i, j, m, n, A);
+    if (retval != PlasmaSuccess) {
+        plasma_error("plasma_desc_general_init() failed");
+        return retval;
+    }
+    A->type = PlasmaGeneralBand;
+    A->uplo = uplo;
+    A->kl = kl;
+    A->ku = ku;
+    if (uplo == PlasmaGeneral) {
+        A->klt = 1 + (i + kl + mb - 1) / mb - i / mb;
+        A->kut = 1 + (i + ku + kl + nb - 1) / nb - i / nb;
+    }
+    else if (uplo == PlasmaUpper) {
+        A->klt = 1;
+        A->kut = 1 + (i + ku + nb - 1) / nb - i / nb;
+    }
+    else {
+        A->klt = 1 + (i + kl + mb - 1) / mb - i / mb;
+        A->kut = 1;
+    }
+    return PlasmaSuccess;
+}
+
+int plasma_desc_general_band_create(plasma_enum_t precision,
+                                    plasma_enum_t uplo, int mb, int nb,
+                                    int lm, int ln, int i, int j, int m,
+                                    int n, int kl, int ku, plasma_desc_t *A)
+{
+    if (plasma_context_self() == NULL) {
+        plasma_error("PLASMA not initialized");
+        return PlasmaErrorNotInitialized;
+    }
+    int retval = plasma_desc_general_band_init(precision, uplo, NULL, mb, nb,
+                                               lm, ln, i, j, m, n, kl, ku, A);
+    if (retval != PlasmaSuccess) {
+        plasma_error("plasma_desc_general_band_init() failed");
+        return retval;
+    }
+    if (plasma_desc_check(*A) != PlasmaSuccess) {
+        plasma_error("plasma_desc_check() failed");
+        return PlasmaErrorIllegalValue;
+    }
+    return desc_allocate(A);
+}
+
+/* Checks of a band descriptor, with the reference's messages and order. */
+int plasma_desc_general_band_check(plasma_desc_t A)
+{
+    if (A.precision != PlasmaRealFloat && A.precision != PlasmaRealDouble &&
+        A.precision != PlasmaComplexFloat &&
+        A.precision != PlasmaComplexDouble) {
+        plasma_error("invalid matrix type");
+        return PlasmaErrorIllegalValue;
+    }
+    if (A.mb <= 0 || A.nb <= 0) {
+        plasma_error("negative tile dimension");
+        return PlasmaErrorIllegalValue;
+    }
+    if (A.m < 0 || A.n < 0) {
+        plasma_error("negative matrix dimension");
+        return PlasmaErrorIllegalValue;
+    }
+    if (A.gn < A.n) {
+        plasma_error("invalid leading column dimensions");
+        return PlasmaErrorIllegalValue;
+    }
+    if ((A.uplo == PlasmaGeneral &&
+         A.gm < A.mb * ((2 * A.kl + A.ku + A.mb) / A.mb)) ||
+        (A.uplo == PlasmaUpper && A.gm < A.mb * ((A.ku + A.mb) / A.mb)) ||
+        (A.uplo == PlasmaLower && A.gm < A.mb * ((A.kl + A.mb) / A.mb))) {
+        plasma_error("invalid leading row dimensions");
+        return PlasmaErrorIllegalValue;
+    }
+    if ((A.i > 0 && A.i >= A.gm) || (A.j > 0 && A.j >= A.gn)) {
+        plasma_error("beginning of the matrix out of bounds");
+        return PlasmaErrorIllegalValue;
+    }
+    if (A.j + A.n > A.gn) {
+        plasma_error("submatrix out of bounds");
+        return PlasmaErrorIllegalValue;
+    }
+    if (A.i % A.mb != 0 || A.j % A.nb != 0) {
+        plasma_error("submatrix not aligned to a tile");
+        return PlasmaErrorIllegalValue;
+    }
+    if (A.kl + 1 > A.m || A.ku + 1 > A.n) {
+        plasma_error("band width larger than matrix dimension");
+        return PlasmaErrorIllegalValue;
+    }
+    return PlasmaSuccess;
+}
+
 int plasma_desc_check(plasma_desc_t A)
 {
     if (A.type == PlasmaGeneral || A.type == PlasmaUpper ||
         A.type == PlasmaLower)
         return plasma_desc_general_check(A);
+    if (A.type == PlasmaGeneralBand)
+        return plasma_desc_general_band_check(A);
     plasma_error("invalid matrix type");
     return PlasmaErrorIllegalValue;
 }

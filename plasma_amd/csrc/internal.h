@@ -231,6 +231,15 @@ void plasma_pxsbinv(plasma_enum_t uplo, plasma_desc_t A, const void *inv,
 void plasma_pxpotrs_sb(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
                        const void *V, plasma_sequence_t *sequence,
                        plasma_request_t *request);
+/* Band Cholesky and its solve on a band descriptor (PlasmaGeneralBand);
+ * inv: the diagonal-tile inverses (A.mt tiles), NULL in pxpbtrf for a
+ * private buffer.  plasma_band_tiles: tile rows of a band column. */
+int plasma_band_tiles(plasma_enum_t uplo, plasma_desc_t A);
+void plasma_pxpbtrf(plasma_enum_t uplo, plasma_desc_t A, void *inv,
+                    plasma_sequence_t *sequence, plasma_request_t *request);
+void plasma_pxpbtrs(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
+                    const void *inv, plasma_sequence_t *sequence,
+                    plasma_request_t *request);
 /* Either form for a factor solved many times: rsolve_prepare computes it
  * into one pool block (NULL: none, the plain sweeps run), rsolve solves. */
 void *plasma_rsolve_prepare(plasma_enum_t uplo, plasma_desc_t A,

@@ -51,6 +51,42 @@ __global__ __launch_bounds__(NTHREADS) void layout_kernel(
     }
 }
 
+// LAPACK band array AB <-> the tiles of a band descriptor (reference
+// core_blas/core_zlacpy_band.c, lapack2tile_band / tile2lapack_band): A(i,j)
+// of the band (lower: 0 <= i - j <= kd, upper: 0 <= j - i <= kd) is
+// AB[off + i - j + j ldab] with off = kd for upper storage, 0 for lower.
+// Tile (m, n) of the band, one workgroup each; towards the tiles every
+// entry outside the band (or past the matrix) is written as zero, so the
+// factorization finds zeros there.
+template <typename T, bool TO_TILES>
+__global__ __launch_bounds__(NTHREADS) void band_layout_kernel(
+    plasma_desc_t A, T *pAB, int ldab, int kd, int upper, const int *status)
+{
+    if (block_should_stop(status)) return;
+    const int bt = upper ? A.kut : A.klt;
+    const int n = blockIdx.x / bt, d = blockIdx.x % bt;
+    const int m = upper ? n - d : n + d;
+    if (m < 0 || m >= A.mt || n >= A.nt) return;
+    T *tile = (T *)plasma_tile_addr(A, m, n);
+    const int ldt = plasma_tile_mmain(A, m);
+    const int off = upper ? kd : 0;
+    // a tile of the last tile column is only n % nb wide in the band
+    // storage (the next tile follows it): no column past the view
+    const int ncols = plasma_tile_nview(A, n);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int c = wave; c < ncols; c += NTHREADS / 64) {
+        const int j = n * A.nb + c;
+        for (int r = lane; r < A.mb; r += 64) {
+            const int i = m * A.mb + r;
+            const int dist = upper ? j - i : i - j;
+            const bool in = i < A.m && j < A.n && dist >= 0 && dist <= kd;
+            T *ab = pAB + (ptrdiff_t)(off + i - j) + (ptrdiff_t)j * ldab;
+            if (TO_TILES) tile[r + (size_t)c * ldt] = in ? *ab : T(0);
+            else if (in)  *ab = tile[r + (size_t)c * ldt];
+        }
+    }
+}
+
 // Tile-wise precision conversion between two general descriptors of the
 // same shape and tiling.  FP64 -> FP32 rounds to nearest; an entry with
 // x > FLT_MAX or x < -FLT_MAX (so also +-Inf, but not NaN) sets the status
@@ -203,5 +239,38 @@ extern "C" int plasma_kern_geadd(double alpha, plasma_desc_t A, double beta,
                            B, 0, status);
     else
         return PlasmaErrorNotSupported;
+    return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
+}
+
+extern "C" int plasma_kern_band_layout(plasma_desc_t A, void *pAB, int ldab,
+                                       int kd, int upper, int to_tiles,
+                                       const int *status, hipStream_t s)
+{
+    const int bt = upper ? A.kut : A.klt;
+    const long nblocks = (long)A.nt * bt;
+    if (nblocks <= 0) return 0;
+    if (nblocks > 0x7fffffffL) return PlasmaErrorNotSupported;
+    const dim3 grid((unsigned)nblocks), block(NTHREADS);
+    if (A.precision == PlasmaRealDouble) {
+        if (to_tiles)
+            hipLaunchKernelGGL((band_layout_kernel<double, true>), grid, block,
+                               0, s, A, (double *)pAB, ldab, kd, upper,
+                               status);
+        else
+            hipLaunchKernelGGL((band_layout_kernel<double, false>), grid,
+                               block, 0, s, A, (double *)pAB, ldab, kd, upper,
+                               status);
+    }
+    else if (A.precision == PlasmaRealFloat) {
+        if (to_tiles)
+            hipLaunchKernelGGL((band_layout_kernel<float, true>), grid, block,
+                               0, s, A, (float *)pAB, ldab, kd, upper, status);
+        else
+            hipLaunchKernelGGL((band_layout_kernel<float, false>), grid, block,
+                               0, s, A, (float *)pAB, ldab, kd, upper, status);
+    }
+    else {
+        return PlasmaErrorNotSupported;
+    }
     return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
 }

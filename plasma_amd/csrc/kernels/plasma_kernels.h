@@ -24,6 +24,13 @@
 extern "C" {
 #endif
 
+/* LAPACK band array (device) <-> a band descriptor's tiles, kd the band
+ * width of the stored triangle (upper: superdiagonals).  to_tiles writes
+ * every element of the band's tiles (zero outside the band). */
+int plasma_kern_band_layout(plasma_desc_t A, void *pAB, int ldab, int kd,
+                            int upper, int to_tiles, const int *status,
+                            hipStream_t s);
+
 /* Fill n ints with v (a kernel, never hipMemset: safe under graph capture). */
 int plasma_kern_fill_int(int *p, int v, int n, hipStream_t s);
 /* *dst = *src on the stream (one int; a kernel, capture-safe). */
@@ -134,6 +141,12 @@ int plasma_kern_sweep_update(plasma_enum_t uplo, int backward,
                              plasma_desc_t A, plasma_desc_t B, int k,
                              const void *scratch, const int *status,
                              hipStream_t s);
+/* the same, updating only block rows [k + 1, mhi) forward or [mlo, k)
+ * backward (band factors: the rows within the band of tile k) */
+int plasma_kern_sweep_update_range(plasma_enum_t uplo, int backward,
+                                   plasma_desc_t A, plasma_desc_t B, int k,
+                                   int mlo, int mhi, const void *scratch,
+                                   const int *status, hipStream_t s);
 
 /*
  * Sweeps for repeated solves with one factor.  plasma_kern_scale_panels

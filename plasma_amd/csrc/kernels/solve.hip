@@ -106,7 +106,7 @@ __global__ __launch_bounds__(NTHREADS) void sweep_diag_kernel(
 template <typename T>
 __global__ __launch_bounds__(NTHREADS) void sweep_update_kernel(
     plasma_desc_t A, plasma_desc_t B, int upper, int backward, int k,
-    int ntiles, int rchunks, int cchunks, const T *scratch,
+    int mlo, int ntiles, int rchunks, int cchunks, const T *scratch,
     const int *status)
 {
     typedef GemmNT<T, SW_BN, SW_BM, SW_KC, 16, 32> G;
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(NTHREADS) void sweep_update_kernel(
         return;
     }
 
-    const int m = backward ? (int)(blockIdx.x / rchunks)
+    const int m = backward ? mlo + (int)(blockIdx.x / rchunks)
                            : k + 1 + (int)(blockIdx.x / rchunks);
     const int r0 = (blockIdx.x % rchunks) * SW_BM;
     const int mv = plasma_tile_mview(B, m);
@@ -175,7 +175,7 @@ struct SweepDecode {
     plasma_desc_t A, B;
     const T *inv;
     T *S;
-    int k, backward, upper, diag, bpt, cbpt;
+    int k, backward, upper, diag, bpt, cbpt, mlo;
     __device__ bool operator()(int b, tg::Job<T> &j) const
     {
         const int ncb = B.nt * cbpt;
@@ -209,7 +209,7 @@ struct SweepDecode {
         }
         else {
             // forward: B(m) -= L(m,k) S, m > k; backward: B(m) -= L(k,m)^T S
-            const int m = backward ? t : k + 1 + t;
+            const int m = backward ? mlo + t : k + 1 + t;
             const int mv = plasma_tile_mview(B, m);
             if (r0 >= mv) return false;
             View<T> X;
@@ -264,7 +264,7 @@ int launch_sweep_diag(int backward, plasma_desc_t A, plasma_desc_t B, int k,
         const int bpt = (A.mb + tg::BM - 1) / tg::BM;
         const int cbpt = (B.nb + tg::BM - 1) / tg::BM;
         SweepDecode<T> d{A, B, (const T *)inv, (T *)scratch, k, backward, 0,
-                         1, bpt, cbpt};
+                         1, bpt, cbpt, 0};
         const int njobs = bpt * B.nt * cbpt;
         return backward ? launch_wide<T, false, false>(d, njobs, status, s)
                         : launch_wide<T, false, true>(d, njobs, status, s);
@@ -278,12 +278,14 @@ int launch_sweep_diag(int backward, plasma_desc_t A, plasma_desc_t B, int k,
     return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
 }
 
+// Rows updated: forward m in [k + 1, mhi), backward m in [mlo, k).
 template <typename T>
 int launch_sweep_update(plasma_enum_t uplo, int backward, plasma_desc_t A,
-                        plasma_desc_t B, int k, const void *scratch,
-                        const int *status, hipStream_t s)
+                        plasma_desc_t B, int k, int mlo, int mhi,
+                        const void *scratch, const int *status, hipStream_t s)
 {
-    int ntiles = backward ? k : B.mt - k - 1;
+    int ntiles = backward ? k - mlo : mhi - k - 1;
+    if (ntiles < 0) ntiles = 0;
     const int rchunks = (B.mb + SW_BM - 1) / SW_BM;
     const int cchunks = (min(B.nb, B.n) + SW_BN - 1) / SW_BN;
     if (B.n >= WIDE_RHS) {
@@ -293,7 +295,7 @@ int launch_sweep_update(plasma_enum_t uplo, int backward, plasma_desc_t A,
         const int bpt = (A.mb + tg::BM - 1) / tg::BM;
         const int cbpt = (B.nb + tg::BM - 1) / tg::BM;
         SweepDecode<T> d{A, B, nullptr, (T *)scratch, k, backward, upper, 0,
-                         bpt, cbpt};
+                         bpt, cbpt, mlo};
         const int njobs = bpt * B.nt * cbpt * ntiles;
         // X has unit row stride for forward-lower and backward-upper
         const int err = (backward == upper)
@@ -305,7 +307,7 @@ int launch_sweep_update(plasma_enum_t uplo, int backward, plasma_desc_t A,
     hipLaunchKernelGGL(sweep_update_kernel<T>,
                        dim3(ntiles * rchunks + rchunks, B.nt * cchunks),
                        dim3(NTHREADS), 0, s, A, B, uplo == PlasmaUpper,
-                       backward, k, ntiles, rchunks, cchunks,
+                       backward, k, mlo, ntiles, rchunks, cchunks,
                        (const T *)scratch, status);
     return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
 }
@@ -487,11 +489,30 @@ extern "C" int plasma_kern_sweep_update(plasma_enum_t uplo, int backward,
 {
     if (A.precision != B.precision) return PlasmaErrorIllegalValue;
     if (A.precision == PlasmaRealDouble)
-        return launch_sweep_update<double>(uplo, backward, A, B, k, scratch,
-                                           status, s);
+        return launch_sweep_update<double>(uplo, backward, A, B, k, 0, B.mt,
+                                           scratch, status, s);
     if (A.precision == PlasmaRealFloat)
-        return launch_sweep_update<float>(uplo, backward, A, B, k, scratch,
-                                          status, s);
+        return launch_sweep_update<float>(uplo, backward, A, B, k, 0, B.mt,
+                                          scratch, status, s);
+    return PlasmaErrorNotSupported;
+}
+
+extern "C" int plasma_kern_sweep_update_range(plasma_enum_t uplo,
+                                              int backward, plasma_desc_t A,
+                                              plasma_desc_t B, int k, int mlo,
+                                              int mhi, const void *scratch,
+                                              const int *status,
+                                              hipStream_t s)
+{
+    if (A.precision != B.precision) return PlasmaErrorIllegalValue;
+    mlo = max(mlo, 0);
+    mhi = min(mhi, B.mt);
+    if (A.precision == PlasmaRealDouble)
+        return launch_sweep_update<double>(uplo, backward, A, B, k, mlo, mhi,
+                                           scratch, status, s);
+    if (A.precision == PlasmaRealFloat)
+        return launch_sweep_update<float>(uplo, backward, A, B, k, mlo, mhi,
+                                          scratch, status, s);
     return PlasmaErrorNotSupported;
 }
 

@@ -689,10 +689,12 @@ def roofline(prof, kernel, peak, traffic=None):
 
 
 def split_roofline(r, split):
-    """The FP32 update's roofline entry: FP32-algorithmic flops against the
-    FP32 matrix peak; with a split update also the 16-bit MFMA work it
-    issues (six bf16 or three fp16 products per FP32 product) against the
-    16-bit dense peak."""
+    """The FP32 update's roofline entry: FP32-algorithmic flops per launch
+    over the launch time.  On FP32 MFMA the peak is the FP32 matrix peak.
+    A split update runs each FP32 product as p 16-bit products (six bf16 or
+    three fp16), so its ceiling is the 16-bit dense peak / p in
+    FP32-algorithmic flops, and frac is also the fraction of the 16-bit
+    peak its issued MFMA work reaches (reported beside it)."""
     if r is None:
         return None
     if not split:
@@ -707,6 +709,12 @@ def split_roofline(r, split):
                         "fragments, v_mfma_f32_16x16x32_f16")
     products = SPLIT_PRODUCTS[split]
     work = r["achieved"] * 1e12 * products
+    r["peak"] = round(HALF_MFMA_PEAK / products / 1e12, 1)
+    r["peak_basis"] = ("%s dense MFMA peak %.0f TFLOP/s / %d products per "
+                       "FP32 product" % (split, HALF_MFMA_PEAK / 1e12,
+                                         products))
+    r["frac"] = round(r["achieved"] * 1e12 * products / HALF_MFMA_PEAK, 4)
+    r["fp32_mfma_peak_frac"] = round(r["achieved"] * 1e12 / FP32_MFMA_PEAK, 4)
     r["mfma_work"] = {"unit": "TFLOP/s", "precision": split,
                       "products_per_fp32_product": products,
                       "achieved": round(work / 1e12, 1),
@@ -804,7 +812,7 @@ def main():
                "tflops": round(fl / (ms_g * 1e-3) / 1e12, 3),
                "ms": round(ms_g, 3), "residual_eps": round(res_g, 4),
                "roofline": roofline(
-                   prof_g, "trailing_update_kernel<double> (FP64 MFMA "
+                   prof_g, "upd64_kernel (FP64 MFMA "
                    "syrk+gemm)", FP64_MFMA_PEAK),
                "kernels": classes(prof_g, args.steps)}
 
@@ -847,7 +855,7 @@ def main():
                        "ms": round(ms_p, 3), "residual_eps": round(res_p, 4),
                        "profiled_ms": round(pms_p, 3),
                        "roofline": roofline(
-                           prof_p, "trailing_update_kernel<double> (FP64 "
+                           prof_p, "upd64_kernel (FP64 "
                            "MFMA syrk+gemm)", FP64_MFMA_PEAK),
                        "kernels": classes(prof_p, args.steps)},
         "speedup_vs_plain": round(ms_p / ms_m, 3),

@@ -1,0 +1,58 @@
+/*
+ * plasma_c.h -- single complex (Hermitian) Cholesky solve path.
+ *
+ * Signatures are the reference's c forms (generated from
+ * include/plasma_z.h:272-292, :398-430, :631-646, :667-671).  A is
+ * Hermitian positive definite;
+ * potrf computes A = L L^H (PlasmaLower) or A = U^H U (PlasmaUpper), using
+ * the real part of A's diagonal, and leaves a real diagonal in the factor
+ * (LAPACK cpotrf).  Elements are interleaved (re, im) pairs,
+ * plasma_complex32_t.  Memory rules, return codes and the tile forms are
+ * those of plasma_d.h.
+ */
+#ifndef PLASMA_C_H
+#define PLASMA_C_H
+
+#include "plasma_async.h"
+#include "plasma_descriptor.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* reference compute/zpotrf.c:74 */
+int plasma_cpotrf(plasma_enum_t uplo, int n, plasma_complex32_t *pA, int lda);
+/* reference compute/zpotrs.c:77 */
+int plasma_cpotrs(plasma_enum_t uplo, int n, int nrhs,
+                  plasma_complex32_t *pA, int lda,
+                  plasma_complex32_t *pB, int ldb);
+/* reference compute/zposv.c:90 */
+int plasma_cposv(plasma_enum_t uplo, int n, int nrhs,
+                 plasma_complex32_t *pA, int lda,
+                 plasma_complex32_t *pB, int ldb);
+
+/* reference compute/zpotrf.c:206 */
+void plasma_omp_cpotrf(plasma_enum_t uplo, plasma_desc_t A,
+                       plasma_sequence_t *sequence, plasma_request_t *request);
+/* reference compute/zpotrs.c:226 */
+void plasma_omp_cpotrs(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
+                       plasma_sequence_t *sequence, plasma_request_t *request);
+/* reference compute/zposv.c:244 */
+void plasma_omp_cposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
+                      plasma_sequence_t *sequence, plasma_request_t *request);
+
+/* reference compute/zge2desc.c:27, zdesc2ge.c, ztr2desc.c, zdesc2tr.c */
+void plasma_omp_cge2desc(plasma_complex32_t *pA, int lda, plasma_desc_t A,
+                         plasma_sequence_t *sequence, plasma_request_t *request);
+void plasma_omp_cdesc2ge(plasma_desc_t A, plasma_complex32_t *pA, int lda,
+                         plasma_sequence_t *sequence, plasma_request_t *request);
+void plasma_omp_ctr2desc(plasma_complex32_t *pA, int lda, plasma_desc_t A,
+                         plasma_sequence_t *sequence, plasma_request_t *request);
+void plasma_omp_cdesc2tr(plasma_desc_t A, plasma_complex32_t *pA, int lda,
+                         plasma_sequence_t *sequence, plasma_request_t *request);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PLASMA_C_H */

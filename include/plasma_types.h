@@ -67,6 +67,11 @@ enum {
     PlasmaConstUnknown  = INT_MIN
 };
 
+/* Complex element types of the z / c routines (reference
+ * include/plasma_types.h:170-171): interleaved (re, im) pairs. */
+typedef float  _Complex plasma_complex32_t;
+typedef double _Complex plasma_complex64_t;
+
 /* Return codes (reference include/plasma_types.h:123-137). */
 enum {
     PlasmaSuccess             = 0,

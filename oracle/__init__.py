@@ -69,6 +69,17 @@ def load():
             "ora_dtrsm": (i, [i, i, i, i, i, i, d, p, i, p, i, i]),
             "ora_sgemm": (i, [i, i, i, i, i, f, p, i, p, i, f, p, i, i]),
             "ora_strsm": (i, [i, i, i, i, i, i, f, p, i, p, i, i]),
+            "ora_zpotrf": (i, [i, i, p, i, i]),
+            "ora_cpotrf": (i, [i, i, p, i, i]),
+            "ora_zposv": (i, [i, i, i, p, i, p, i, i]),
+            "ora_cposv": (i, [i, i, i, p, i, p, i, i]),
+            "ora_zpotrs": (i, [i, i, i, p, i, p, i, i]),
+            "ora_cpotrs": (i, [i, i, i, p, i, p, i, i]),
+            "ora_zcposv": (i, [i, i, i, p, i, p, i, p, i, pi, i]),
+            "ora_zlag2c": (i, [i, i, p, i, p, i]),
+            "ora_clag2z": (None, [i, i, p, i, p, i]),
+            "ora_zlanhe_inf": (d, [i, i, p, i]),
+            "ora_dzamax": (None, [i, i, p, i, p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(lib, name)
@@ -290,3 +301,79 @@ def trsm(side, uplo, trans, diag, alpha, a, b, nb, dtype=np.float64):
             b.ctypes.data, _ld(b), nb)
     assert rc == 0, rc
     return b
+
+
+# ---------------------------------------------------------------------------
+# Complex precisions: dtype np.complex128 (z) or np.complex64 (c).
+
+def _zc(dtype):
+    return "z" if np.dtype(dtype) == np.complex128 else "c"
+
+
+def zpotrf(uplo, a, nb, dtype=np.complex128):
+    """Hermitian tile Cholesky (reference compute/pzpotrf.c); returns
+    (factor array, info), the input is not modified."""
+    a = _f(a, dtype).copy(order="F")
+    info = getattr(load(), "ora_%spotrf" % _zc(dtype))(
+        uplo, a.shape[0], a.ctypes.data, _ld(a), nb)
+    return a, info
+
+
+def zposv(uplo, a, b, nb, dtype=np.complex128):
+    a = _f(a, dtype).copy(order="F")
+    b = _f(b, dtype).copy(order="F")
+    info = getattr(load(), "ora_%sposv" % _zc(dtype))(
+        uplo, a.shape[0], b.shape[1], a.ctypes.data, _ld(a), b.ctypes.data,
+        _ld(b), nb)
+    return a, b, info
+
+
+def zpotrs(uplo, l, b, nb, dtype=np.complex128):
+    l = _f(l, dtype)
+    b = _f(b, dtype).copy(order="F")
+    info = getattr(load(), "ora_%spotrs" % _zc(dtype))(
+        uplo, l.shape[0], b.shape[1], l.ctypes.data, _ld(l), b.ctypes.data,
+        _ld(b), nb)
+    return b, info
+
+
+def zcposv(uplo, a, b, nb):
+    """Mixed-precision Hermitian solve (reference compute/zcposv.c);
+    returns (X, iter, info)."""
+    a = _f(a, np.complex128)
+    b = _f(b, np.complex128)
+    x = np.zeros_like(b, order="F")
+    it = ctypes.c_int(0)
+    info = load().ora_zcposv(uplo, a.shape[0], b.shape[1], a.ctypes.data,
+                             _ld(a), b.ctypes.data, _ld(b), x.ctypes.data,
+                             _ld(x), ctypes.byref(it), nb)
+    return x, it.value, info
+
+
+def zlag2c(a):
+    a = _f(a, np.complex128)
+    s = np.zeros(a.shape, dtype=np.complex64, order="F")
+    info = load().ora_zlag2c(a.shape[0], a.shape[1], a.ctypes.data, _ld(a),
+                             s.ctypes.data, _ld(s))
+    return s, info
+
+
+def clag2z(s):
+    s = _f(s, np.complex64)
+    a = np.zeros(s.shape, dtype=np.complex128, order="F")
+    load().ora_clag2z(s.shape[0], s.shape[1], s.ctypes.data, _ld(s),
+                      a.ctypes.data, _ld(a))
+    return a
+
+
+def zlanhe_inf(uplo, a):
+    a = _f(a, np.complex128)
+    return load().ora_zlanhe_inf(uplo, a.shape[0], a.ctypes.data, _ld(a))
+
+
+def dzamax(a):
+    a = _f(a, np.complex128)
+    v = np.zeros(a.shape[1])
+    load().ora_dzamax(a.shape[0], a.shape[1], a.ctypes.data, _ld(a),
+                      v.ctypes.data)
+    return v

@@ -13,6 +13,7 @@
  */
 #define _GNU_SOURCE
 #include <dlfcn.h>
+#include <complex.h>
 #include <float.h>
 #include <math.h>
 #include <omp.h>
@@ -23,7 +24,8 @@
 
 #include "plasma_oracle.h"
 
-enum { ORA_NOTRANS = 111, ORA_TRANS = 112, ORA_LEFT = 141, ORA_RIGHT = 142,
+enum { ORA_NOTRANS = 111, ORA_TRANS = 112, ORA_CONJTRANS = 113,
+       ORA_LEFT = 141, ORA_RIGHT = 142,
        ORA_NONUNIT = 131, ORA_UNIT = 132 };
 
 static int ora_backend = ORA_BLAS_BUILTIN;
@@ -764,4 +766,199 @@ int ora_dpotrf_adjoint(int uplo, int n, const double *pL, int ldl,
     free(G);
     free(Ab);
     return 0;
+}
+
+/* ---------------------------------------------------------------------
+ * Complex precisions (reference compute/zpotrf.c, zpotrs.c, zposv.c,
+ * zcposv.c, zlag2c.c, clag2z.c): the tile algorithms of ora_ztile.inc on
+ * double complex (z) and float complex (c) data.
+ * --------------------------------------------------------------------- */
+#define CPLX double complex
+#define RPART double
+#define ZF(name) z_##name
+#define CONJ conj
+#define CREAL creal
+#define CIMAG cimag
+#define CABS cabs
+#include "ora_ztile.inc"
+#undef CPLX
+#undef RPART
+#undef ZF
+#undef CONJ
+#undef CREAL
+#undef CIMAG
+#undef CABS
+
+#define CPLX float complex
+#define RPART float
+#define ZF(name) c_##name
+#define CONJ conjf
+#define CREAL crealf
+#define CIMAG cimagf
+#define CABS cabsf
+#include "ora_ztile.inc"
+#undef CPLX
+#undef RPART
+#undef ZF
+#undef CONJ
+#undef CREAL
+#undef CIMAG
+#undef CABS
+
+int ora_zpotrf(int uplo, int n, void *A, int lda, int nb)
+{
+    return z_solve(uplo, n, 0, A, lda, NULL, 1, nb, 1);
+}
+int ora_cpotrf(int uplo, int n, void *A, int lda, int nb)
+{
+    return c_solve(uplo, n, 0, A, lda, NULL, 1, nb, 1);
+}
+int ora_zposv(int uplo, int n, int nrhs, void *A, int lda, void *B, int ldb,
+              int nb)
+{
+    return z_solve(uplo, n, nrhs, A, lda, B, ldb, nb, 1);
+}
+int ora_cposv(int uplo, int n, int nrhs, void *A, int lda, void *B, int ldb,
+              int nb)
+{
+    return c_solve(uplo, n, nrhs, A, lda, B, ldb, nb, 1);
+}
+/* potrs: A holds the factor (only read) */
+int ora_zpotrs(int uplo, int n, int nrhs, const void *L, int lda, void *B,
+               int ldb, int nb)
+{
+    return z_solve(uplo, n, nrhs, (double complex *)L, lda, B, ldb, nb, 0);
+}
+int ora_cpotrs(int uplo, int n, int nrhs, const void *L, int lda, void *B,
+               int ldb, int nb)
+{
+    return c_solve(uplo, n, nrhs, (float complex *)L, lda, B, ldb, nb, 0);
+}
+
+/* LAPACK zlag2c: 1 when the real or imaginary part of an entry is outside
+ * [-FLT_MAX, FLT_MAX] (conversion stops there) */
+int ora_zlag2c(int m, int n, const void *vA, int lda, void *vAs, int ldas)
+{
+    const double complex *A = vA;
+    float complex *As = vAs;
+    const double rmax = (double)FLT_MAX;
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < m; i++) {
+            const double complex v = A[(size_t)j * lda + i];
+            const double re = creal(v), im = cimag(v);
+            if (re < -rmax || re > rmax || im < -rmax || im > rmax) return 1;
+            As[(size_t)j * ldas + i] = (float)re + (float)im * I;
+        }
+    return 0;
+}
+
+void ora_clag2z(int m, int n, const void *vAs, int ldas, void *vA, int lda)
+{
+    const float complex *As = vAs;
+    double complex *A = vA;
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < m; i++) {
+            const float complex v = As[(size_t)j * ldas + i];
+            A[(size_t)j * lda + i] = (double)crealf(v) + (double)cimagf(v) * I;
+        }
+}
+
+double ora_zlanhe_inf(int uplo, int n, const void *A, int lda)
+{
+    return z_lanhe_inf(uplo, n, A, lda);
+}
+
+void ora_dzamax(int m, int n, const void *A, int lda, double *values)
+{
+    z_amax_cols(m, n, A, lda, values);
+}
+
+/* reference compute/zcposv.c:363-539, as ora_dsposv with the complex
+ * pieces: zlanhe, zlag2c / clag2z, the single complex tile factor and
+ * sweeps, the double complex Hermitian residual and dzamax */
+int ora_zcposv(int uplo, int n, int nrhs, const void *vA, int lda,
+               const void *vB, int ldb, void *vX, int ldx, int *iter, int nb)
+{
+    typedef double complex zc_t;
+    typedef float complex cc_t;
+    const zc_t *pA = vA, *pB = vB;
+    zc_t *pX = vX;
+    const int itermax = 30;
+    *iter = 0;
+    if (n == 0 || nrhs == 0) return 0;
+    zc_t *A = malloc((size_t)n * n * sizeof(zc_t));
+    zc_t *B = malloc((size_t)n * nrhs * sizeof(zc_t));
+    zc_t *X = malloc((size_t)n * nrhs * sizeof(zc_t));
+    zc_t *R = malloc((size_t)n * nrhs * sizeof(zc_t));
+    cc_t *As = malloc((size_t)n * n * sizeof(cc_t));
+    cc_t *Xs = malloc((size_t)n * nrhs * sizeof(cc_t));
+    double *xn = malloc(nrhs * sizeof(double)), *rn = malloc(nrhs * sizeof(double));
+    int info = 0;
+    for (int j = 0; j < n; j++)
+        memcpy(A + (size_t)j * n, pA + (size_t)j * lda, n * sizeof(zc_t));
+    for (int j = 0; j < nrhs; j++)
+        memcpy(B + (size_t)j * n, pB + (size_t)j * ldb, n * sizeof(zc_t));
+
+    const double eps = ldexp(1.0, -53);
+    const double anorm = z_lanhe_inf(uplo, n, A, n);
+    ora_desc_t tAs, tXs;
+    ora_desc_create(&tAs, 0, sizeof(cc_t), nb, n, n);
+    ora_desc_create(&tXs, 0, sizeof(cc_t), nb, n, nrhs);
+
+    if (ora_zlag2c(n, nrhs, B, n, Xs, n) || ora_zlag2c(n, n, A, n, As, n)) {
+        info = 1;
+        goto done;
+    }
+    {
+        volatile int status = 0;
+        c_lapack2tile(0, As, n, tAs, 1);
+        #pragma omp parallel num_threads(ora_nthreads())
+        #pragma omp master
+        {
+            c_pxpotrf(uplo, tAs, &status);
+        }
+        if (status) { info = status; goto done; }
+    }
+    c_lapack2tile(0, Xs, n, tXs, 1);
+    if ((info = c_sweeps(uplo, tAs, tXs))) goto done;
+    c_lapack2tile(0, Xs, n, tXs, 0);
+    ora_clag2z(n, nrhs, Xs, n, X, n);
+
+    memcpy(R, B, (size_t)n * nrhs * sizeof(zc_t));
+    z_hemm_res(uplo, n, nrhs, A, n, X, n, R, n);
+    z_amax_cols(n, nrhs, X, n, xn);
+    z_amax_cols(n, nrhs, R, n, rn);
+    const double cte = anorm * eps * sqrt((double)n);
+    if (conv(rn, xn, nrhs, cte)) {
+        *iter = 0;
+        goto done;
+    }
+    for (int it = 0; it < itermax; it++) {
+        if (ora_zlag2c(n, nrhs, R, n, Xs, n)) { info = 1; goto done; }
+        c_lapack2tile(0, Xs, n, tXs, 1);
+        if ((info = c_sweeps(uplo, tAs, tXs))) goto done;
+        c_lapack2tile(0, Xs, n, tXs, 0);
+        ora_clag2z(n, nrhs, Xs, n, R, n);
+        for (size_t e = 0; e < (size_t)n * nrhs; e++) X[e] += R[e];
+        memcpy(R, B, (size_t)n * nrhs * sizeof(zc_t));
+        z_hemm_res(uplo, n, nrhs, A, n, X, n, R, n);
+        z_amax_cols(n, nrhs, X, n, xn);
+        z_amax_cols(n, nrhs, R, n, rn);
+        if (conv(rn, xn, nrhs, cte)) {
+            *iter = it + 1;
+            goto done;
+        }
+    }
+    *iter = -itermax - 1;
+    memcpy(X, B, (size_t)n * nrhs * sizeof(zc_t));
+    info = z_solve(uplo, n, nrhs, A, n, X, n, nb, 1);
+
+done:
+    if (info == 0)
+        for (int j = 0; j < nrhs; j++)
+            memcpy(pX + (size_t)j * ldx, X + (size_t)j * n, n * sizeof(zc_t));
+    ora_desc_destroy(&tAs);
+    ora_desc_destroy(&tXs);
+    free(A); free(B); free(X); free(R); free(As); free(Xs); free(xn); free(rn);
+    return info;
 }

@@ -118,4 +118,23 @@ void ora_tile2lapack(int uplo, int m, int n, const double *tiles, double *A,
 }
 #endif
 
+/* Complex precisions (void * = interleaved double complex for z, float
+ * complex for c): Hermitian Cholesky, solves and the mixed solver. */
+int ora_zpotrf(int uplo, int n, void *A, int lda, int nb);
+int ora_cpotrf(int uplo, int n, void *A, int lda, int nb);
+int ora_zposv(int uplo, int n, int nrhs, void *A, int lda, void *B, int ldb,
+              int nb);
+int ora_cposv(int uplo, int n, int nrhs, void *A, int lda, void *B, int ldb,
+              int nb);
+int ora_zpotrs(int uplo, int n, int nrhs, const void *L, int lda, void *B,
+               int ldb, int nb);
+int ora_cpotrs(int uplo, int n, int nrhs, const void *L, int lda, void *B,
+               int ldb, int nb);
+int ora_zcposv(int uplo, int n, int nrhs, const void *A, int lda,
+               const void *B, int ldb, void *X, int ldx, int *iter, int nb);
+int ora_zlag2c(int m, int n, const void *A, int lda, void *As, int ldas);
+void ora_clag2z(int m, int n, const void *As, int ldas, void *A, int lda);
+double ora_zlanhe_inf(int uplo, int n, const void *A, int lda);
+void ora_dzamax(int m, int n, const void *A, int lda, double *values);
+
 #endif

@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libplasma_amd.so")
 
 # --- enums (reference include/plasma_types.h:49-164) ---------------------
 PlasmaRealFloat, PlasmaRealDouble = 2, 3
+PlasmaComplexFloat, PlasmaComplexDouble = 4, 5
 PlasmaNoTrans, PlasmaTrans, PlasmaConjTrans = 111, 112, 113
 PlasmaUpper, PlasmaLower, PlasmaGeneral = 121, 122, 123
 PlasmaGeneralBand = 124
@@ -162,6 +163,31 @@ for _p, _t in (("d", "double"), ("s", "float")):
         "plasma_omp_%strsm" % _p: (None, [c_int, c_int, c_int, c_int, _S, _D,
                                           _D, _PS, _PR]),
     })
+for _p in ("z", "c"):
+    PROTOTYPES.update({
+        "plasma_%spotrf" % _p: (c_int, [c_int, c_int, c_void_p, c_int]),
+        "plasma_%spotrs" % _p: (c_int, [c_int, c_int, c_int, c_void_p, c_int,
+                                        c_void_p, c_int]),
+        "plasma_%sposv" % _p: (c_int, [c_int, c_int, c_int, c_void_p, c_int,
+                                       c_void_p, c_int]),
+        "plasma_omp_%spotrf" % _p: (None, [c_int, _D, _PS, _PR]),
+        "plasma_omp_%spotrs" % _p: (None, [c_int, _D, _D, _PS, _PR]),
+        "plasma_omp_%sposv" % _p: (None, [c_int, _D, _D, _PS, _PR]),
+        "plasma_omp_%sge2desc" % _p: (None, [c_void_p, c_int, _D, _PS, _PR]),
+        "plasma_omp_%sdesc2ge" % _p: (None, [_D, c_void_p, c_int, _PS, _PR]),
+        "plasma_omp_%str2desc" % _p: (None, [c_void_p, c_int, _D, _PS, _PR]),
+        "plasma_omp_%sdesc2tr" % _p: (None, [_D, c_void_p, c_int, _PS, _PR]),
+    })
+PROTOTYPES.update({
+    "plasma_zcposv": (c_int, [c_int, c_int, c_int, c_void_p, c_int, c_void_p,
+                              c_int, c_void_p, c_int, _PI]),
+    "plasma_omp_zcposv": (None, [c_int, _D, _D, _D, _D, _D, _D, c_void_p,
+                                 c_void_p, c_void_p, _PI, _PS, _PR]),
+    "plasma_zlag2c": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_int]),
+    "plasma_clag2z": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_int]),
+    "plasma_omp_zlag2c": (None, [_D, _D, _PS, _PR]),
+    "plasma_omp_clag2z": (None, [_D, _D, _PS, _PR]),
+})
 PROTOTYPES["plasma_norm_work_doubles"] = (c_size_t, [_D])
 PROTOTYPES.update({
     "plasma_dsposv": (c_int, [c_int, c_int, c_int, c_void_p, c_int, c_void_p,
@@ -287,6 +313,14 @@ def potrs_adjoint(uplo, n, nrhs, a, lda, x, ldx, xbar, ldxbar, abar, ldabar,
 def dsposv(uplo, n, nrhs, a, lda, b, ldb, x, ldx):
     it = c_int(0)
     rc = load().plasma_dsposv(uplo, n, nrhs, _ptr(a), lda, _ptr(b), ldb,
+                              _ptr(x), ldx, ctypes.byref(it))
+    return rc, it.value
+
+
+def zcposv(uplo, n, nrhs, a, lda, b, ldb, x, ldx):
+    """plasma_zcposv: (return code, iterations)."""
+    it = c_int(0)
+    rc = load().plasma_zcposv(uplo, n, nrhs, _ptr(a), lda, _ptr(b), ldb,
                               _ptr(x), ldx, ctypes.byref(it))
     return rc, it.value
 

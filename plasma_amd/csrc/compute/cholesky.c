@@ -1,7 +1,8 @@
 /*
  * cholesky.c -- the LAPACK-layout and tile-async drivers of the Cholesky
- * solve path: plasma_{d,s}potrf, plasma_{d,s}potrs, plasma_{d,s}posv and
- * their plasma_omp_* forms.
+ * solve path: plasma_{d,s,z,c}potrf, plasma_{d,s,z,c}potrs,
+ * plasma_{d,s,z,c}posv and their plasma_omp_* forms (complex: Hermitian,
+ * A = L L^H or U^H U).
  *
  * Argument checks, return codes and quick returns follow the reference
  * drivers (compute/zpotrf.c:74-150 and :206-246, compute/zpotrs.c:77-172
@@ -392,3 +393,5 @@ void plasma_omp_##P##posv(plasma_enum_t uplo, plasma_desc_t A,                \
 
 DEFINE_CHOL(d, double, PlasmaRealDouble)
 DEFINE_CHOL(s, float, PlasmaRealFloat)
+DEFINE_CHOL(z, plasma_complex64_t, PlasmaComplexDouble)
+DEFINE_CHOL(c, plasma_complex32_t, PlasmaComplexFloat)

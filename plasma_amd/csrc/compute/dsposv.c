@@ -1,7 +1,10 @@
 /*
  * dsposv.c -- mixed-precision Cholesky solve with FP64 iterative refinement
- * (plasma_dsposv, plasma_omp_dsposv) and the FP64 <-> FP32 conversions
- * (plasma_dlag2s, plasma_slag2d and their tile forms).
+ * (plasma_dsposv, plasma_omp_dsposv; complex: plasma_zcposv,
+ * plasma_omp_zcposv) and the FP64 <-> FP32 conversions (plasma_dlag2s,
+ * plasma_slag2d, plasma_zlag2c, plasma_clag2z and their tile forms).  One
+ * body serves both precision pairs: the kernels dispatch on the
+ * descriptors' precision (Hermitian residual and norms for complex data).
  *
  * Algorithm and constants follow the reference (compute/zcposv.c:363-539
  * generated as dsposv):
@@ -79,8 +82,9 @@ static int residual_step(plasma_enum_t uplo, plasma_desc_t A,
                          plasma_request_t *request)
 {
     const double n = A.n, nrhs = X.n;
+    const double cf = plasma_is_complex(A.precision) ? 4.0 : 1.0;
     int slot;
-    plasma_prof_begin(s, PlasmaProfMixed, 2.0 * n * n * nrhs,
+    plasma_prof_begin(s, PlasmaProfMixed, cf * 2.0 * n * n * nrhs,
                       n * (n + 1) / 2 * 8.0 + 3.0 * n * nrhs * 8.0, &slot);
     int err =
         issue(plasma_kern_lacpy(B, R, st, s), sequence, request) ||
@@ -109,11 +113,13 @@ static int lag2_step(plasma_desc_t S, plasma_desc_t D, int *st, hipStream_t s,
     return err;
 }
 
-void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
-                       plasma_desc_t X, plasma_desc_t As, plasma_desc_t Xs,
-                       plasma_desc_t R, double *W, double *Rnorm,
-                       double *Xnorm, int *iter, plasma_sequence_t *sequence,
-                       plasma_request_t *request)
+/* hi / lo: the working and the factor precision (RealDouble / RealFloat
+ * or ComplexDouble / ComplexFloat) */
+static void omp_mixed(plasma_enum_t hi, plasma_enum_t lo, plasma_enum_t uplo,
+                      plasma_desc_t A, plasma_desc_t B, plasma_desc_t X,
+                      plasma_desc_t As, plasma_desc_t Xs, plasma_desc_t R,
+                      double *W, double *Rnorm, double *Xnorm, int *iter,
+                      plasma_sequence_t *sequence, plasma_request_t *request)
 {
     const int itermax = 30;
     (void)W;
@@ -149,14 +155,13 @@ void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
     }
     if (A.n == 0 || B.n == 0)
         return;
-    if (A.precision != PlasmaRealDouble || B.precision != PlasmaRealDouble ||
-        X.precision != PlasmaRealDouble || R.precision != PlasmaRealDouble ||
-        As.precision != PlasmaRealFloat || Xs.precision != PlasmaRealFloat ||
+    if (A.precision != hi || B.precision != hi || X.precision != hi ||
+        R.precision != hi || As.precision != lo || Xs.precision != lo ||
         A.mb != A.nb || As.mb != A.mb || B.mb != A.mb || X.mb != A.mb ||
         R.mb != A.mb || Xs.mb != A.mb || A.m != A.n || B.m != A.m ||
         X.m != A.m || R.m != A.m || Xs.m != A.m || As.m != A.m ||
         B.n != X.n || R.n != X.n || Xs.n != X.n) {
-        plasma_error("descriptors do not match the dsposv contract");
+        plasma_error("descriptors do not match the mixed solver's contract");
         plasma_request_fail(sequence, request, PlasmaErrorIllegalValue);
         return;
     }
@@ -261,8 +266,30 @@ done:
     plasma_dev_free(work);
 }
 
-int plasma_dsposv(plasma_enum_t uplo, int n, int nrhs, double *pA, int lda,
-                  double *pB, int ldb, double *pX, int ldx, int *iter)
+void plasma_omp_dsposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
+                       plasma_desc_t X, plasma_desc_t As, plasma_desc_t Xs,
+                       plasma_desc_t R, double *W, double *Rnorm,
+                       double *Xnorm, int *iter, plasma_sequence_t *sequence,
+                       plasma_request_t *request)
+{
+    omp_mixed(PlasmaRealDouble, PlasmaRealFloat, uplo, A, B, X, As, Xs, R, W,
+              Rnorm, Xnorm, iter, sequence, request);
+}
+
+void plasma_omp_zcposv(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
+                       plasma_desc_t X, plasma_desc_t As, plasma_desc_t Xs,
+                       plasma_desc_t R, double *W, double *Rnorm,
+                       double *Xnorm, int *iter, plasma_sequence_t *sequence,
+                       plasma_request_t *request)
+{
+    omp_mixed(PlasmaComplexDouble, PlasmaComplexFloat, uplo, A, B, X, As, Xs,
+              R, W, Rnorm, Xnorm, iter, sequence, request);
+}
+
+static int mixed_driver(plasma_enum_t hi, plasma_enum_t lo,
+                        plasma_enum_t uplo, int n, int nrhs, void *pA,
+                        int lda, void *pB, int ldb, void *pX, int ldx,
+                        int *iter)
 {
     plasma_context_t *ctx = plasma_context_self();
     if (ctx == NULL) {
@@ -299,37 +326,37 @@ int plasma_dsposv(plasma_enum_t uplo, int n, int nrhs, double *pA, int lda,
 
     /* reference zcposv.c:160-161: tuned with the single-precision table */
     if (ctx->tuning)
-        plasma_tune_potrf(ctx, PlasmaRealFloat, n);
+        plasma_tune_potrf(ctx, lo, n);
     const int nb = ctx->nb;
     plasma_desc_t A, B, X, R, As, Xs;
     plasma_desc_t *all[6] = {&A, &B, &X, &R, &As, &Xs};
     int created = 0, retval;
-    retval = plasma_desc_triangular_create(PlasmaRealDouble, uplo, nb, nb, n,
-                                           n, 0, 0, n, n, &A);
+    retval = plasma_desc_triangular_create(hi, uplo, nb, nb, n, n, 0, 0, n,
+                                           n, &A);
     if (retval == PlasmaSuccess) created++;
     if (retval == PlasmaSuccess) {
-        retval = plasma_desc_general_create(PlasmaRealDouble, nb, nb, n, nrhs,
-                                            0, 0, n, nrhs, &B);
+        retval = plasma_desc_general_create(hi, nb, nb, n, nrhs, 0, 0, n,
+                                            nrhs, &B);
         if (retval == PlasmaSuccess) created++;
     }
     if (retval == PlasmaSuccess) {
-        retval = plasma_desc_general_create(PlasmaRealDouble, nb, nb, n, nrhs,
-                                            0, 0, n, nrhs, &X);
+        retval = plasma_desc_general_create(hi, nb, nb, n, nrhs, 0, 0, n,
+                                            nrhs, &X);
         if (retval == PlasmaSuccess) created++;
     }
     if (retval == PlasmaSuccess) {
-        retval = plasma_desc_general_create(PlasmaRealDouble, nb, nb, n, nrhs,
-                                            0, 0, n, nrhs, &R);
+        retval = plasma_desc_general_create(hi, nb, nb, n, nrhs, 0, 0, n,
+                                            nrhs, &R);
         if (retval == PlasmaSuccess) created++;
     }
     if (retval == PlasmaSuccess) {
-        retval = plasma_desc_triangular_create(PlasmaRealFloat, uplo, nb, nb,
-                                               n, n, 0, 0, n, n, &As);
+        retval = plasma_desc_triangular_create(lo, uplo, nb, nb, n, n, 0, 0,
+                                               n, n, &As);
         if (retval == PlasmaSuccess) created++;
     }
     if (retval == PlasmaSuccess) {
-        retval = plasma_desc_general_create(PlasmaRealFloat, nb, nb, n, nrhs,
-                                            0, 0, n, nrhs, &Xs);
+        retval = plasma_desc_general_create(lo, nb, nb, n, nrhs, 0, 0, n,
+                                            nrhs, &Xs);
         if (retval == PlasmaSuccess) created++;
     }
     if (retval != PlasmaSuccess) {
@@ -348,8 +375,8 @@ int plasma_dsposv(plasma_enum_t uplo, int n, int nrhs, double *pA, int lda,
      * translated through the staging path of the layout routines */
     plasma_pxlapack2desc(uplo, pA, lda, A, &sequence, &request);
     plasma_pxlapack2desc(PlasmaGeneral, pB, ldb, B, &sequence, &request);
-    plasma_omp_dsposv(uplo, A, B, X, As, Xs, R, NULL, NULL, NULL, iter,
-                      &sequence, &request);
+    omp_mixed(hi, lo, uplo, A, B, X, As, Xs, R, NULL, NULL, NULL, iter,
+              &sequence, &request);
     plasma_pxdesc2lapack(PlasmaGeneral, X, pX, ldx, &sequence, &request);
     int status = plasma_sequence_sync(&sequence);
     for (int i = 0; i < 6; i++)
@@ -357,12 +384,29 @@ int plasma_dsposv(plasma_enum_t uplo, int n, int nrhs, double *pA, int lda,
     return status;
 }
 
+int plasma_dsposv(plasma_enum_t uplo, int n, int nrhs, double *pA, int lda,
+                  double *pB, int ldb, double *pX, int ldx, int *iter)
+{
+    return mixed_driver(PlasmaRealDouble, PlasmaRealFloat, uplo, n, nrhs, pA,
+                        lda, pB, ldb, pX, ldx, iter);
+}
+
+int plasma_zcposv(plasma_enum_t uplo, int n, int nrhs,
+                  plasma_complex64_t *pA, int lda, plasma_complex64_t *pB,
+                  int ldb, plasma_complex64_t *pX, int ldx, int *iter)
+{
+    return mixed_driver(PlasmaComplexDouble, PlasmaComplexFloat, uplo, n,
+                        nrhs, pA, lda, pB, ldb, pX, ldx, iter);
+}
+
 /***************************************************************************
  * Precision conversions (reference compute/zlag2c.c:59-231,
- * compute/clag2z.c:58-226).
+ * compute/clag2z.c:58-226).  hi / lo as in the mixed solver; a complex
+ * conversion checks both parts against FLT_MAX (LAPACK zlag2c).
  */
-void plasma_omp_dlag2s(plasma_desc_t A, plasma_desc_t As,
-                       plasma_sequence_t *sequence, plasma_request_t *request)
+static void omp_lag2(plasma_enum_t hi, plasma_enum_t lo, int to_lo,
+                     plasma_desc_t A, plasma_desc_t As,
+                     plasma_sequence_t *sequence, plasma_request_t *request)
 {
     plasma_context_t *ctx = plasma_context_self();
     if (ctx == NULL) {
@@ -371,10 +415,10 @@ void plasma_omp_dlag2s(plasma_desc_t A, plasma_desc_t As,
         return;
     }
     if (plasma_desc_check(A) != PlasmaSuccess ||
-        plasma_desc_check(As) != PlasmaSuccess ||
-        A.precision != PlasmaRealDouble || As.precision != PlasmaRealFloat ||
-        A.mt != As.mt || A.nt != As.nt || A.mb != As.mb || A.nb != As.nb) {
-        plasma_error("invalid A or As");
+        plasma_desc_check(As) != PlasmaSuccess || A.precision != hi ||
+        As.precision != lo || A.mt != As.mt || A.nt != As.nt ||
+        A.mb != As.mb || A.nb != As.nb) {
+        plasma_error(to_lo ? "invalid A or As" : "invalid As or A");
         plasma_request_fail(sequence, request, PlasmaErrorIllegalValue);
         return;
     }
@@ -386,39 +430,40 @@ void plasma_omp_dlag2s(plasma_desc_t A, plasma_desc_t As,
     if (A.m == 0 || A.n == 0 || sequence->status != PlasmaSuccess)
         return;
     int *st = plasma_sequence_dstatus(sequence, request);
-    issue(plasma_kern_lag2(A, As, st, ctx->stream), sequence, request);
+    if (to_lo)
+        issue(plasma_kern_lag2(A, As, st, ctx->stream), sequence, request);
+    else
+        issue(plasma_kern_lag2(As, A, st, ctx->stream), sequence, request);
+}
+
+void plasma_omp_dlag2s(plasma_desc_t A, plasma_desc_t As,
+                       plasma_sequence_t *sequence, plasma_request_t *request)
+{
+    omp_lag2(PlasmaRealDouble, PlasmaRealFloat, 1, A, As, sequence, request);
 }
 
 void plasma_omp_slag2d(plasma_desc_t As, plasma_desc_t A,
                        plasma_sequence_t *sequence, plasma_request_t *request)
 {
-    plasma_context_t *ctx = plasma_context_self();
-    if (ctx == NULL) {
-        plasma_error("PLASMA not initialized");
-        plasma_request_fail(sequence, request, PlasmaErrorIllegalValue);
-        return;
-    }
-    if (plasma_desc_check(A) != PlasmaSuccess ||
-        plasma_desc_check(As) != PlasmaSuccess ||
-        A.precision != PlasmaRealDouble || As.precision != PlasmaRealFloat ||
-        A.mt != As.mt || A.nt != As.nt || A.mb != As.mb || A.nb != As.nb) {
-        plasma_error("invalid As or A");
-        plasma_request_fail(sequence, request, PlasmaErrorIllegalValue);
-        return;
-    }
-    if (sequence == NULL || request == NULL) {
-        plasma_error("NULL sequence or request");
-        plasma_request_fail(sequence, request, PlasmaErrorIllegalValue);
-        return;
-    }
-    if (A.m == 0 || A.n == 0 || sequence->status != PlasmaSuccess)
-        return;
-    int *st = plasma_sequence_dstatus(sequence, request);
-    issue(plasma_kern_lag2(As, A, st, ctx->stream), sequence, request);
+    omp_lag2(PlasmaRealDouble, PlasmaRealFloat, 0, A, As, sequence, request);
 }
 
-static int lag2_driver(int to_single, int m, int n, void *pA, int lda,
-                       void *pAs, int ldas)
+void plasma_omp_zlag2c(plasma_desc_t A, plasma_desc_t As,
+                       plasma_sequence_t *sequence, plasma_request_t *request)
+{
+    omp_lag2(PlasmaComplexDouble, PlasmaComplexFloat, 1, A, As, sequence,
+             request);
+}
+
+void plasma_omp_clag2z(plasma_desc_t As, plasma_desc_t A,
+                       plasma_sequence_t *sequence, plasma_request_t *request)
+{
+    omp_lag2(PlasmaComplexDouble, PlasmaComplexFloat, 0, A, As, sequence,
+             request);
+}
+
+static int lag2_driver(plasma_enum_t hi, plasma_enum_t lo, int to_lo, int m,
+                       int n, void *pA, int lda, void *pAs, int ldas)
 {
     plasma_context_t *ctx = plasma_context_self();
     if (ctx == NULL) {
@@ -434,7 +479,7 @@ static int lag2_driver(int to_single, int m, int n, void *pA, int lda,
         return -2;
     }
     /* argument 4 is the source's leading dimension, 6 the target's */
-    int ld_src = to_single ? lda : ldas, ld_dst = to_single ? ldas : lda;
+    int ld_src = to_lo ? lda : ldas, ld_dst = to_lo ? ldas : lda;
     if (ld_src < imax(1, m)) {
         plasma_error("illegal value of the source leading dimension");
         return -4;
@@ -447,12 +492,10 @@ static int lag2_driver(int to_single, int m, int n, void *pA, int lda,
         return PlasmaSuccess;
     const int nb = ctx->nb;
     plasma_desc_t A, As;
-    int retval = plasma_desc_general_create(PlasmaRealDouble, nb, nb, m, n, 0,
-                                            0, m, n, &A);
+    int retval = plasma_desc_general_create(hi, nb, nb, m, n, 0, 0, m, n, &A);
     if (retval != PlasmaSuccess)
         return retval;
-    retval = plasma_desc_general_create(PlasmaRealFloat, nb, nb, m, n, 0, 0,
-                                        m, n, &As);
+    retval = plasma_desc_general_create(lo, nb, nb, m, n, 0, 0, m, n, &As);
     if (retval != PlasmaSuccess) {
         plasma_desc_destroy(&A);
         return retval;
@@ -461,16 +504,16 @@ static int lag2_driver(int to_single, int m, int n, void *pA, int lda,
     plasma_request_t request;
     plasma_sequence_init(&sequence);
     plasma_request_init(&request);
-    if (to_single) {
+    if (to_lo) {
         plasma_pxlapack2desc(PlasmaGeneral, pA, lda, A, &sequence, &request);
-        plasma_omp_dlag2s(A, As, &sequence, &request);
+        omp_lag2(hi, lo, 1, A, As, &sequence, &request);
         plasma_pxdesc2lapack(PlasmaGeneral, As, pAs, ldas, &sequence,
                              &request);
     }
     else {
         plasma_pxlapack2desc(PlasmaGeneral, pAs, ldas, As, &sequence,
                              &request);
-        plasma_omp_slag2d(As, A, &sequence, &request);
+        omp_lag2(hi, lo, 0, A, As, &sequence, &request);
         plasma_pxdesc2lapack(PlasmaGeneral, A, pA, lda, &sequence, &request);
     }
     int status = plasma_sequence_sync(&sequence);
@@ -481,10 +524,26 @@ static int lag2_driver(int to_single, int m, int n, void *pA, int lda,
 
 int plasma_dlag2s(int m, int n, double *pA, int lda, float *pAs, int ldas)
 {
-    return lag2_driver(1, m, n, pA, lda, pAs, ldas);
+    return lag2_driver(PlasmaRealDouble, PlasmaRealFloat, 1, m, n, pA, lda,
+                       pAs, ldas);
 }
 
 int plasma_slag2d(int m, int n, float *pAs, int ldas, double *pA, int lda)
 {
-    return lag2_driver(0, m, n, pA, lda, pAs, ldas);
+    return lag2_driver(PlasmaRealDouble, PlasmaRealFloat, 0, m, n, pA, lda,
+                       pAs, ldas);
+}
+
+int plasma_zlag2c(int m, int n, plasma_complex64_t *pA, int lda,
+                  plasma_complex32_t *pAs, int ldas)
+{
+    return lag2_driver(PlasmaComplexDouble, PlasmaComplexFloat, 1, m, n, pA,
+                       lda, pAs, ldas);
+}
+
+int plasma_clag2z(int m, int n, plasma_complex32_t *pAs, int ldas,
+                  plasma_complex64_t *pA, int lda)
+{
+    return lag2_driver(PlasmaComplexDouble, PlasmaComplexFloat, 0, m, n, pA,
+                       lda, pAs, ldas);
 }

@@ -202,3 +202,5 @@ void plasma_omp_##P##desc2tr(plasma_desc_t A, TYPE *pA, int lda,              \
 
 DEFINE_LAYOUT(d, double, PlasmaRealDouble)
 DEFINE_LAYOUT(s, float, PlasmaRealFloat)
+DEFINE_LAYOUT(z, plasma_complex64_t, PlasmaComplexDouble)
+DEFINE_LAYOUT(c, plasma_complex32_t, PlasmaComplexFloat)

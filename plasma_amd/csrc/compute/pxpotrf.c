@@ -587,6 +587,10 @@ void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A, void *inv,
 {
     if (sequence->status != PlasmaSuccess)
         return;
+    if (plasma_is_complex(A.precision)) {
+        plasma_pzpotrf(uplo, A, inv, sequence, request);
+        return;
+    }
     plasma_context_t *ctx = plasma_context_self();
     int *st = plasma_sequence_dstatus(sequence, request);
     hipStream_t s0 = ctx->stream;
@@ -678,6 +682,10 @@ void plasma_pxpotrs(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
 {
     if (sequence->status != PlasmaSuccess)
         return;
+    if (plasma_is_complex(A.precision)) {
+        plasma_pzpotrs(uplo, A, B, inv, sequence, request);
+        return;
+    }
     void *own_inv = NULL;
     if (inv == NULL) {
         plasma_context_t *ctx = plasma_context_self();
@@ -878,6 +886,9 @@ void *plasma_rsolve_prepare(plasma_enum_t uplo, plasma_desc_t A,
                             const void *inv, plasma_sequence_t *sequence,
                             plasma_request_t *request)
 {
+    /* complex factors are solved by the plain sweeps */
+    if (plasma_is_complex(A.precision))
+        return NULL;
     const int sb = plasma_sweep_tiles(A) > 0;
     void *form = plasma_dev_alloc(sb ? plasma_sbinv_bytes(A)
                                      : plasma_mpanel_bytes(A));

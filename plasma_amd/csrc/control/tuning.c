@@ -6,7 +6,8 @@
  * compute/zpotrf.c:105-106, zposv.c:130-131 and, in single precision,
  * zcposv.c:160-161), and leaves the context untouched when the tables are
  * absent.  Here the table is plain text, written by the tuner
- * (plasma_amd/tune.py) from measurements on the GPU:
+ * (plasma_amd/tune.py) from measurements on the GPU (precision d, s, z or
+ * c; the complex factorization reads only nb):
  *
  *     # routine precision n_max nb update_depth lookahead [mode [split]]
  *     potrf d 8192 256 2 1 graph
@@ -64,7 +65,8 @@ int plasma_tuning_load(const char *path)
                                split);
         if (line[0] == '#' || got < 6 || strcmp(routine, "potrf") != 0)
             continue;
-        if ((prec != 'd' && prec != 's') || n_max < 1 || nb < 1 ||
+        if ((prec != 'd' && prec != 's' && prec != 'z' && prec != 'c') ||
+            n_max < 1 || nb < 1 ||
             depth < 1 || depth > 4 || la < 0 || la > 4)
             continue;
         int m = 0;
@@ -87,7 +89,10 @@ int plasma_tuning_load(const char *path)
                 continue;
         }
         plasma_tune_row_t *r = &ctx->tune[ctx->ntune++];
-        r->precision = prec == 'd' ? PlasmaRealDouble : PlasmaRealFloat;
+        r->precision = prec == 'd'   ? PlasmaRealDouble
+                       : prec == 's' ? PlasmaRealFloat
+                       : prec == 'z' ? PlasmaComplexDouble
+                                     : PlasmaComplexFloat;
         r->n_max = n_max;
         r->nb = nb;
         r->depth = depth;

@@ -202,6 +202,12 @@ void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A, void *inv,
 void plasma_pxpotrs(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
                     const void *inv, plasma_sequence_t *sequence,
                     plasma_request_t *request);
+/* complex descriptors (compute/zchol.c), reached through the two above */
+void plasma_pzpotrf(plasma_enum_t uplo, plasma_desc_t A, void *inv,
+                    plasma_sequence_t *sequence, plasma_request_t *request);
+void plasma_pzpotrs(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t B,
+                    const void *inv, plasma_sequence_t *sequence,
+                    plasma_request_t *request);
 void plasma_pxtrsm_left(plasma_enum_t uplo, plasma_enum_t trans,
                         plasma_desc_t A, plasma_desc_t B, const void *inv,
                         plasma_sequence_t *sequence,

@@ -173,6 +173,10 @@ extern "C" int plasma_kern_lapack2tile(plasma_enum_t region, const void *pA,
                                        int lda, plasma_desc_t A,
                                        const int *status, hipStream_t s)
 {
+    if (plasma_is_complex(A.precision)) {
+        A = plasma_desc_real_view(A);
+        lda *= 2;
+    }
     if (A.precision == PlasmaRealDouble)
         return launch_layout<double, true>(region, (void *)pA, lda, A, status, s);
     if (A.precision == PlasmaRealFloat)
@@ -184,6 +188,10 @@ extern "C" int plasma_kern_tile2lapack(plasma_enum_t region, plasma_desc_t A,
                                        void *pA, int lda, const int *status,
                                        hipStream_t s)
 {
+    if (plasma_is_complex(A.precision)) {
+        A = plasma_desc_real_view(A);
+        lda *= 2;
+    }
     if (A.precision == PlasmaRealDouble)
         return launch_layout<double, false>(region, pA, lda, A, status, s);
     if (A.precision == PlasmaRealFloat)
@@ -194,6 +202,12 @@ extern "C" int plasma_kern_tile2lapack(plasma_enum_t region, plasma_desc_t A,
 extern "C" int plasma_kern_lag2(plasma_desc_t src, plasma_desc_t dst,
                                 int *status, hipStream_t s)
 {
+    // complex: both parts converted, an overflow of either sets the status
+    // (LAPACK zlag2c, reference core_zlag2c.c:48-79)
+    if (plasma_is_complex(src.precision) != plasma_is_complex(dst.precision))
+        return PlasmaErrorIllegalValue;
+    src = plasma_desc_real_view(src);
+    dst = plasma_desc_real_view(dst);
     const long ntiles = (long)src.mt * src.nt;
     if (ntiles == 0) return 0;
     if (src.precision == PlasmaRealDouble && dst.precision == PlasmaRealFloat)
@@ -211,6 +225,8 @@ extern "C" int plasma_kern_lag2(plasma_desc_t src, plasma_desc_t dst,
 extern "C" int plasma_kern_lacpy(plasma_desc_t A, plasma_desc_t B,
                                  const int *status, hipStream_t s)
 {
+    A = plasma_desc_real_view(A);
+    B = plasma_desc_real_view(B);
     const long ntiles = (long)B.mt * B.nt;
     if (ntiles == 0) return 0;
     if (A.precision == PlasmaRealDouble)
@@ -228,6 +244,9 @@ extern "C" int plasma_kern_geadd(double alpha, plasma_desc_t A, double beta,
                                  plasma_desc_t B, const int *status,
                                  hipStream_t s)
 {
+    // real alpha and beta: a complex descriptor is added part by part
+    A = plasma_desc_real_view(A);
+    B = plasma_desc_real_view(B);
     const long ntiles = (long)B.mt * B.nt;
     if (ntiles == 0) return 0;
     if (A.precision == PlasmaRealDouble)

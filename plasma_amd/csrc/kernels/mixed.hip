@@ -387,7 +387,11 @@ extern "C" int plasma_kern_lansy_inf(plasma_enum_t uplo, plasma_desc_t A,
     const int nblk = (int)((ld + NTHREADS - 1) / NTHREADS);
     double *partial = work + (size_t)A.mt * ld;
     const long ntiles = (long)A.mt * (A.mt + 1) / 2;
-    if (A.precision == PlasmaRealDouble)
+    if (plasma_is_complex(A.precision)) {
+        const int err = plasma_kern_zlanhe_tiles(uplo, A, work, status, s);
+        if (err) return err;
+    }
+    else if (A.precision == PlasmaRealDouble)
         hipLaunchKernelGGL(lansy_tile_kernel<double>, dim3((unsigned)ntiles),
                            dim3(NTHREADS), 0, s, A, uplo == PlasmaUpper, work,
                            status);
@@ -408,6 +412,8 @@ extern "C" int plasma_kern_amax_cols(plasma_desc_t A, double *values,
                                      const int *status, hipStream_t s)
 {
     if (A.n == 0) return 0;
+    if (plasma_is_complex(A.precision))
+        return plasma_kern_zamax_cols(A, values, status, s);
     if (A.precision == PlasmaRealDouble)
         hipLaunchKernelGGL(amax_cols_kernel<double>, dim3(A.n), dim3(NTHREADS),
                            0, s, A, values, status);
@@ -426,6 +432,8 @@ extern "C" int plasma_kern_symm_residual(plasma_enum_t uplo, plasma_desc_t A,
 {
     if (A.precision != X.precision || A.precision != R.precision)
         return PlasmaErrorIllegalValue;
+    if (plasma_is_complex(A.precision))
+        return plasma_kern_zhemm_residual(uplo, A, X, R, status, s);
     if (A.precision == PlasmaRealDouble)
         return launch_symm<double>(uplo, A, X, R, work, work_bytes, status,
                                    s);

@@ -402,6 +402,77 @@ int plasma_kern_conv_test(const double *rnorm, const double *xnorm, int nrhs,
                           const double *anorm, double eps_sqrtn, int *flag,
                           const int *status, hipStream_t s);
 
+/*
+ * Complex precisions (kernels/zchol.hip), PlasmaComplexDouble and
+ * PlasmaComplexFloat, lower-equivalent L = U^H for PlasmaUpper.
+ *
+ * zpotrf_diag: factor diagonal tile k (LAPACK zpotrf semantics; info
+ * nb*k + j + 1 into the status word) and write W_kk = L(k,k)^{-1} to
+ * inv + k * mb * mb (ld mb, zeros above).  ztrtri_diag: W_kk of every
+ * diagonal tile of an already factored A.
+ * zpanel: P(m) = L(m,k) W_kk^H, m > k, into slot m-k-1 of panel_buf
+ * ((mt-1) slots of mb x nb, ld mb).  zupdate: copies the panels into
+ * L(m,k) and applies L(m,n) -= P(m) P(n)^H, k < n <= m.
+ * zsweep_diag: S = W_kk B(k,:) (forward) or W_kk^H B(k,:) (backward) into
+ * S (mb x B.n, ld mb); zsweep_update: B(m,:) -= L(m,k) S (m > k) or
+ * L(k,m)^H S (m < k), and B(k,:) = S.
+ * zhemm_residual: R <- R - A X, A Hermitian in its uplo triangle.
+ * zlanhe_tiles: the row-sum work array of plasma_kern_lansy_inf;
+ * zamax_cols: values[j] = max_i |Re a_ij| + |Im a_ij|.
+ */
+int plasma_kern_zpotrf_diag(plasma_enum_t uplo, plasma_desc_t A, int k,
+                            void *inv, int *status, hipStream_t s);
+int plasma_kern_ztrtri_diag(plasma_enum_t uplo, plasma_desc_t A, void *inv,
+                            int *status, hipStream_t s);
+int plasma_kern_zpanel(plasma_enum_t uplo, plasma_desc_t A, int k,
+                       const void *inv, void *panel_buf, const int *status,
+                       hipStream_t s);
+int plasma_kern_zupdate(plasma_enum_t uplo, plasma_desc_t A, int k,
+                        const void *panel_buf, const int *status,
+                        hipStream_t s);
+int plasma_kern_zsweep_diag(int backward, plasma_desc_t B, int k,
+                            const void *inv, void *S, const int *status,
+                            hipStream_t s);
+int plasma_kern_zsweep_update(plasma_enum_t uplo, int backward,
+                              plasma_desc_t A, plasma_desc_t B, int k,
+                              const void *S, const int *status,
+                              hipStream_t s);
+int plasma_kern_zhemm_residual(plasma_enum_t uplo, plasma_desc_t A,
+                               plasma_desc_t X, plasma_desc_t R,
+                               const int *status, hipStream_t s);
+int plasma_kern_zlanhe_tiles(plasma_enum_t uplo, plasma_desc_t A,
+                             double *work, const int *status, hipStream_t s);
+int plasma_kern_zamax_cols(plasma_desc_t A, double *values,
+                           const int *status, hipStream_t s);
+
+/* A complex descriptor seen as a real one of twice the rows: element
+ * (i, j) of the complex view is the pair (2i, j), (2i+1, j) of the real
+ * view, tile by tile (interleaved storage, every offset doubled).  Lets the
+ * element-wise kernels (layout, conversion, copy, add) serve complex data;
+ * a real descriptor is returned unchanged. */
+static inline plasma_desc_t plasma_desc_real_view(plasma_desc_t A)
+{
+    if (A.precision != PlasmaComplexDouble &&
+        A.precision != PlasmaComplexFloat)
+        return A;
+    A.precision = A.precision == PlasmaComplexDouble ? PlasmaRealDouble
+                                                     : PlasmaRealFloat;
+    A.mb *= 2;
+    A.gm *= 2;
+    A.i *= 2;
+    A.m *= 2;
+    A.A21 *= 2;
+    A.A12 *= 2;
+    A.A22 *= 2;
+    return A;
+}
+
+static inline int plasma_is_complex(plasma_enum_t precision)
+{
+    return precision == PlasmaComplexDouble ||
+           precision == PlasmaComplexFloat;
+}
+
 #ifdef __cplusplus
 }
 #endif

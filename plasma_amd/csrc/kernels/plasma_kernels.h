@@ -411,8 +411,9 @@ int plasma_kern_conv_test(const double *rnorm, const double *xnorm, int nrhs,
  * inv + k * mb * mb (ld mb, zeros above).  ztrtri_diag: W_kk of every
  * diagonal tile of an already factored A.
  * zpanel: P(m) = L(m,k) W_kk^H, m > k, into slot m-k-1 of panel_buf
- * ((mt-1) slots of mb x nb, ld mb).  zupdate: copies the panels into
- * L(m,k) and applies L(m,n) -= P(m) P(n)^H, k < n <= m.
+ * ((mt-1) slots of mb x nb, ld mb).  zupdate: L(m,n) -= P(m) P(n)^H for
+ * the columns n in [n_first, n_end) (n_first > k), n <= m, and with copy
+ * the panels of step k into L(m,k).
  * zsweep_diag: S = W_kk B(k,:) (forward) or W_kk^H B(k,:) (backward) into
  * S (mb x B.n, ld mb); zsweep_update: B(m,:) -= L(m,k) S (m > k) or
  * L(k,m)^H S (m < k), and B(k,:) = S.
@@ -428,6 +429,7 @@ int plasma_kern_zpanel(plasma_enum_t uplo, plasma_desc_t A, int k,
                        const void *inv, void *panel_buf, const int *status,
                        hipStream_t s);
 int plasma_kern_zupdate(plasma_enum_t uplo, plasma_desc_t A, int k,
+                        int n_first, int n_end, int copy,
                         const void *panel_buf, const int *status,
                         hipStream_t s);
 int plasma_kern_zsweep_diag(int backward, plasma_desc_t B, int k,

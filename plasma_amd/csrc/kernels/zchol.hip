@@ -211,13 +211,21 @@ __device__ __forceinline__ void cprod(CAcc<R, MI, NJ> &acc,
 #pragma unroll
         for (int j = 0; j < NJ; j++) nb[j] = czero<R>();
         if (k0 + 4 < K) fetch(na, nb, k0 + 4);
+        // two passes over the blocks, so consecutive products go to
+        // different accumulators (a product waits for the previous one on
+        // its accumulator)
 #pragma unroll
         for (int i = 0; i < MI; i++)
 #pragma unroll
             for (int j = 0; j < NJ; j++) {
                 acc.re[i][j] = Mfma<R>::mma(a[i].re, b[j].re, acc.re[i][j]);
-                acc.re[i][j] = Mfma<R>::mma(-a[i].im, b[j].im, acc.re[i][j]);
                 acc.im[i][j] = Mfma<R>::mma(a[i].re, b[j].im, acc.im[i][j]);
+            }
+#pragma unroll
+        for (int i = 0; i < MI; i++)
+#pragma unroll
+            for (int j = 0; j < NJ; j++) {
+                acc.re[i][j] = Mfma<R>::mma(-a[i].im, b[j].im, acc.re[i][j]);
                 acc.im[i][j] = Mfma<R>::mma(a[i].im, b[j].re, acc.im[i][j]);
             }
 #pragma unroll
@@ -361,13 +369,14 @@ template <typename R> struct PanelDec {
     }
 };
 
-// Trailing update of step k: L(m,n) -= P(m) P(n)^H for k < n <= m < mt,
-// lower triangle only on diagonal tiles (the herk and gemm tasks of
-// pzpotrf.c:58-78; upper: :100-125).  The first ncopy jobs copy the panels
-// back into L(m,k); the update reads the buffer, not those tiles.
+// Update of step k: L(m,n) -= P(m) P(n)^H for n in [n_first, n_end),
+// n <= m < mt (n_first > k), lower triangle only on diagonal tiles (the
+// herk and gemm tasks of pzpotrf.c:58-78; upper: :100-125).  The first
+// ncopy jobs copy the panels back into L(m,k); the update reads the
+// buffer, not those tiles.
 template <typename R> struct UpdateDec {
     plasma_desc_t A;
-    int upper, k, nbr, ncopy;
+    int upper, k, nbr, ncopy, n_first;
     const cx<R> *P;
 
     __device__ bool job(int b, Job<R> &j) const
@@ -386,7 +395,7 @@ template <typename R> struct UpdateDec {
         }
         else {
             b -= ncopy;
-            trapezoid_decode(b / per, A.mt, k + 1, &m, &n);
+            trapezoid_decode(b / per, A.mt, n_first, &m, &n);
         }
         const int blk = b % per, br = blk % nbr, bc = blk / nbr;
         if (!j.copy && m == n) {
@@ -636,21 +645,27 @@ __global__ __launch_bounds__(NT) void zdiag_kernel(plasma_desc_t A, int upper,
                 return;
             }
         }
-        // Di = D^{-1}: thread c solves column c of D Di = I
-        if (tid < w) {
-            const int c = tid;
-            for (int i = 0; i < c; i++) Di[i][c] = czero<R>();
-            for (int i = c; i < w; i++) {
-                cx<R> s = (i == c) ? cx<R>{R(1), R(0)} : czero<R>();
-                for (int l = c; l < i; l++) {
-                    const cx<R> t = cmul(D[i][l], Di[l][c]);
-                    s.re -= t.re;
-                    s.im -= t.im;
-                }
-                Di[i][c] = cdivide(s, D[i][i]);
-            }
+        // Di = D^{-1} by forward elimination on the identity, every entry
+        // of a step in parallel: row j of Di is final after step j,
+        //     Di(j,:) /= D(j,j);  Di(i,:) -= D(i,j) Di(j,:), i > j
+        for (int e = tid; e < BS * BS; e += NT) {
+            const int r = e % BS, c = e / BS;
+            Di[r][c] = (r == c) ? cx<R>{R(1), R(0)} : czero<R>();
         }
         __syncthreads();
+        for (int j = 0; j < w; j++) {
+            if (tid <= j) Di[j][tid] = cdivide(Di[j][tid], D[j][j]);
+            __syncthreads();
+            for (int e = tid; e < BS * BS; e += NT) {
+                const int r = e % BS, c = e / BS;
+                if (r > j && r < w && c <= j) {
+                    const cx<R> t = cmul(D[r][j], Di[j][c]);
+                    Di[r][c].re -= t.re;
+                    Di[r][c].im -= t.im;
+                }
+            }
+            __syncthreads();
+        }
         for (int e = tid; e < BS * BS; e += NT) {
             const int r = e % BS, c = e / BS;
             if (r < w && c < w) {
@@ -822,20 +837,24 @@ int panel(plasma_enum_t uplo, plasma_desc_t A, int k, const void *inv,
 }
 
 template <typename R>
-int update(plasma_enum_t uplo, plasma_desc_t A, int k, const void *P,
-           const int *status, hipStream_t s)
+int update(plasma_enum_t uplo, plasma_desc_t A, int k, int n_first,
+           int n_end, int copy, const void *P, const int *status,
+           hipStream_t s)
 {
     UpdateDec<R> d;
     d.A = A;
     d.upper = uplo == PlasmaUpper;
     d.k = k;
+    d.n_first = n_first;
     d.nbr = cdiv(A.mb, 64);
     d.P = (const cx<R> *)P;
     const long per = (long)d.nbr * d.nbr;
     const long rest = (long)A.mt - k - 1;
-    d.ncopy = (int)(rest * per);
-    return launch_jobs<R>(d, rest * per + rest * (rest + 1) / 2 * per, status,
-                          s);
+    d.ncopy = copy ? (int)(rest * per) : 0;
+    // tiles of columns [n_first, n_end): column n holds mt - n of them
+    long tiles = 0;
+    for (int n = n_first; n < n_end; n++) tiles += A.mt - n;
+    return launch_jobs<R>(d, d.ncopy + tiles * per, status, s);
 }
 
 template <typename R>
@@ -934,11 +953,18 @@ extern "C" int plasma_kern_zpanel(plasma_enum_t uplo, plasma_desc_t A, int k,
 }
 
 extern "C" int plasma_kern_zupdate(plasma_enum_t uplo, plasma_desc_t A, int k,
+                                   int n_first, int n_end, int copy,
                                    const void *panel_buf, const int *status,
                                    hipStream_t s)
 {
-    if (is_z(A)) return update<double>(uplo, A, k, panel_buf, status, s);
-    if (is_c(A)) return update<float>(uplo, A, k, panel_buf, status, s);
+    if (n_end > A.mt) n_end = A.mt;
+    if (n_first <= k || (n_first >= n_end && !copy)) return 0;
+    if (is_z(A))
+        return update<double>(uplo, A, k, n_first, n_end, copy, panel_buf,
+                              status, s);
+    if (is_c(A))
+        return update<float>(uplo, A, k, n_first, n_end, copy, panel_buf,
+                             status, s);
     return PlasmaErrorNotSupported;
 }
 

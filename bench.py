@@ -38,7 +38,10 @@ the FP64 fallback); the dominant kernel's roofline (the FP32 trailing
 update of the mixed solve; HIP events on the stream it runs on),
 per-kernel-class times, and the CPU baseline: the oracle's dsposv (PLASMA's tile algorithm
 restated, OpenMP tasks, sequential MKL tile kernels) timed on this host on
-a smaller n.
+a smaller n.  Beside the headline: plain FP64 lines at n = 8192 / 16384,
+the batched solver, the Cholesky adjoint, host-pointer rates, and the
+complex precisions (zposv, cposv, zcposv at n = 16384 with the complex
+update's roofline and the oracle's zposv on this host).
 """
 import argparse
 import ctypes
@@ -188,6 +191,12 @@ def parse():
     ap.add_argument("--config2", type=str, default="8192,16384",
                     help="orders of the plain FP64 dpotrf+dpotrs lines on "
                          "G G^T/n + I (tuned nb; empty skips them)")
+    ap.add_argument("--complex-n", type=int, default=16384,
+                    help="order of the complex-precision lines (zposv, "
+                         "cposv, zcposv; 0 skips them)")
+    ap.add_argument("--complex-cpu-n", type=int, default=6144,
+                    help="order of the oracle zposv CPU sample of the "
+                         "complex lines (0 skips it)")
     ap.add_argument("--no-half", action="store_true",
                     help="skip the half-precision-input GMRES-IR solves "
                          "(plasma_dhposv) of the kappa sweep")
@@ -501,6 +510,134 @@ def config2_line(bench, orders, nrhs):
                                  bench.schedule("d", n)))}
         del A0, B0
         torch.cuda.empty_cache()
+    return out
+
+
+def complex_line(bench, n, nrhs, steps, cpu_n, threads):
+    """The complex precisions on a Hermitian positive definite A = G G^H/n
+    + I (G complex N(0,1)), nrhs complex N(0,1) right-hand sides, lower,
+    device-resident: plasma_zposv and plasma_cposv (rates in the
+    reference's complex flop count, zpotrf 4 n^3 / 3 + zpotrs 8 n^2 nrhs,
+    against the FP64 / FP32 matrix peak that their four-real-product MFMA
+    updates run on), plasma_zcposv (same flop count, its speedup over
+    zposv), the update kernel's roofline from a profiled pass, and the
+    oracle's zposv timed on this host at order cpu_n."""
+    import torch
+    pa = bench.pa
+    lib = bench.lib
+    fl = 4.0 * n ** 3 / 3.0 + 8.0 * n * n * nrhs
+    out = {"workload": "Hermitian A = G G^H/n + I (G complex N(0,1)), "
+                       "n=%d, nrhs=%d, lower; B complex N(0,1)" % (n, nrhs)}
+    g = torch.Generator(device="cuda").manual_seed(11)
+
+    def herm(dt):
+        G = torch.randn(n, n, dtype=dt, device="cuda", generator=g)
+        A = (G @ G.conj().T) / n
+        A.diagonal().add_(1.0)
+        # column major on the device: row j of the tensor is column j
+        return A.T.contiguous()
+
+    def resid(Ac, X, B, eps):
+        """the tester's scaled residual in units of the solve's eps"""
+        A = Ac.T.to(torch.complex128)
+        Xt, Bt = X.T.to(torch.complex128), B.T.to(torch.complex128)
+        R = Bt - A @ Xt
+        return (R.abs().sum(1).max() / (n * A.abs().sum(1).max()
+                * Xt.abs().sum(1).max() * eps)).item()
+
+    zms = None
+    for prec, dt, peak in (("z", torch.complex128, FP64_MFMA_PEAK),
+                           ("c", torch.complex64, FP32_MFMA_PEAK)):
+        A0 = herm(dt)
+        B0 = torch.randn(nrhs, n, dtype=dt, device="cuda", generator=g)
+        W, X = torch.empty_like(A0), torch.empty_like(B0)
+        posv = getattr(lib, "plasma_%sposv" % prec)
+        times = []
+        for it in range(steps + 2):
+            W.copy_(A0)
+            X.copy_(B0)
+            torch.cuda.synchronize()
+            if it == steps + 1:
+                bench.prof_reset()
+            t0 = time.perf_counter()
+            rc = posv(pa.PlasmaLower, n, nrhs, W.data_ptr(), n, X.data_ptr(),
+                      n)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            if rc != 0:
+                raise SystemExit("%sposv failed: %d" % (prec, rc))
+            if it == steps + 1:
+                prof = bench.prof_read(1)
+            elif it > 0:
+                times.append(t1 - t0)
+            print("bench: %sposv n=%d step %d done" % (prec, n, it),
+                  file=sys.stderr, flush=True)
+        ms = sum(times) / len(times) * 1e3
+        if prec == "z":
+            zms = ms
+        out[prec + "posv"] = {
+            "ms": round(ms, 3), "tflops": round(fl / (ms * 1e-3) / 1e12, 3),
+            "frac_peak": round(fl / (ms * 1e-3) / peak, 4),
+            "residual_eps": round(resid(A0, X, B0, 2.0 ** -53 if prec == "z"
+                                        else 2.0 ** -24), 4),
+            "roofline": roofline(prof, "cjobs_kernel<UpdateDec> (complex "
+                                 "herk + gemm, four real MFMA products per "
+                                 "complex product)", peak),
+            "kernels": classes(prof, 1)}
+        del A0, B0, W, X
+        torch.cuda.empty_cache()
+    A0 = herm(torch.complex128)
+    B0 = torch.randn(nrhs, n, dtype=torch.complex128, device="cuda",
+                     generator=g)
+    X = torch.empty_like(B0)
+    it_c = ctypes.c_int(0)
+    times = []
+    for it in range(steps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rc = lib.plasma_zcposv(pa.PlasmaLower, n, nrhs, A0.data_ptr(), n,
+                               B0.data_ptr(), n, X.data_ptr(), n,
+                               ctypes.byref(it_c))
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        if rc != 0:
+            raise SystemExit("zcposv failed: %d" % rc)
+        if it > 0:
+            times.append(t1 - t0)
+        print("bench: zcposv n=%d step %d done" % (n, it), file=sys.stderr,
+              flush=True)
+    ms = sum(times) / len(times) * 1e3
+    out["zcposv"] = {"ms": round(ms, 3),
+                     "tflops": round(fl / (ms * 1e-3) / 1e12, 3),
+                     "iterations": it_c.value,
+                     "residual_eps": round(resid(A0, X, B0, 2.0 ** -53), 4),
+                     "speedup_vs_zposv": round(zms / ms, 3)}
+    del A0, B0, X
+    torch.cuda.empty_cache()
+    if cpu_n > 0:
+        import numpy as np
+        import oracle as ora
+        if threads <= 0:
+            threads = min(16, len(os.sched_getaffinity(0)))
+        ora.set_threads(threads)
+        rng = np.random.default_rng(0)
+        G = rng.standard_normal((cpu_n, cpu_n)) + \
+            1j * rng.standard_normal((cpu_n, cpu_n))
+        A = G @ G.conj().T / cpu_n + np.eye(cpu_n)
+        B = rng.standard_normal((cpu_n, nrhs)) + \
+            1j * rng.standard_normal((cpu_n, nrhs))
+        del G
+        t0 = time.perf_counter()
+        _, _, info = ora.zposv(ora.LOWER, A, B, 128)
+        t1 = time.perf_counter()
+        cfl = 4.0 * cpu_n ** 3 / 3.0 + 8.0 * cpu_n * cpu_n * nrhs
+        out["cpu_baseline"] = {
+            "value": round(cfl / (t1 - t0) / 1e12, 4), "unit": "TFLOP/s",
+            "cores": threads, "kind": "port",
+            "sample": "oracle zposv (PLASMA's complex tile algorithm "
+                      "restated, OpenMP tasks, built-in complex tile loops) "
+                      "n=%d nrhs=%d nb=128, one run, %.1f s, info %d"
+                      % (cpu_n, nrhs, t1 - t0, info)}
     return out
 
 
@@ -873,6 +1010,9 @@ def main():
         "adjoint": {"n=%d" % k: adjoint_line(bench, k, args.steps)
                     for k in orders(args.adjoint_n)} or None,
         "config2": config2_line(bench, orders(args.config2), nrhs) or None,
+        "complex": complex_line(bench, args.complex_n, nrhs, args.steps,
+                                args.complex_cpu_n, args.cpu_threads)
+        if args.complex_n > 0 else None,
         "host_pointers": host,
     }
     if args.calibrate_copy > 0:

@@ -63,6 +63,9 @@ def test_dsposv_iterations_and_residual(plasma, nb_guard, uplo, kappa, margin,
     assert 0 <= it <= 30 and 0 <= ito <= 30, (it, ito)
     if margin is not None:
         assert abs(it - ito) <= margin, (it, ito)
+        # both FP64-accurate: they agree to cond * eps
+        err = np.linalg.norm(X - Xo) / np.linalg.norm(Xo)
+        assert err <= 100 * kappa * U.EPS64, err
     assert U.scaled_residual(A, X, B) < 10 * U.EPS64
 
 

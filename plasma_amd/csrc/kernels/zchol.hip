@@ -178,17 +178,24 @@ template <typename R, int MI, int NJ> struct CAcc {
 // acc += X(xr0 + r, k) Y(k, yc0 + c) over k in [0, K), r < 16 MI, c < 16 NJ;
 // rows r >= xrows and columns c >= ycols read as zero.  Lane l loads
 // X(16 i + l % 16, k0 + l / 16) and Y(k0 + l / 16, 16 j + l % 16): the MFMA
-// operand layout of A[l % 16][l / 16] and B[l / 16][l % 16].
+// operand layout of A[l % 16][l / 16] and B[l / 16][l % 16].  The operands
+// of PF steps are in flight at once (a ring of PF register slots, the loop
+// unrolled by PF so every slot index is a constant): one 4-deep step is
+// only 4 MI NJ products, shorter than an L2 round trip.  FP64 (64 cycles
+// per product) gains from three steps in flight: zposv n = 16384 239 ->
+// 207 ms; FP32 did not (cposv 135 -> 140 ms), so it keeps one.
 template <typename R, int MI, int NJ>
 __device__ __forceinline__ void cprod(CAcc<R, MI, NJ> &acc,
                                       const CView<R> &X, int xr0, int xrows,
                                       const CView<R> &Y, int yc0, int ycols,
                                       int K)
 {
+    constexpr int PF = sizeof(R) == 8 ? 3 : 1;
     if (K <= 0) return;
     const int lane = threadIdx.x & 63;
     const int l16 = lane & 15, lk = lane >> 4;
-    cx<R> a[MI], b[NJ];
+    cx<R> a[PF][MI], b[PF][NJ];
+    // operands of the step at k0 (zeros past K: nothing is loaded there)
     auto fetch = [&](cx<R>(&fa)[MI], cx<R>(&fb)[NJ], int k0) {
         const int k = k0 + lk;
         const bool kin = k < K;
@@ -203,35 +210,36 @@ __device__ __forceinline__ void cprod(CAcc<R, MI, NJ> &acc,
             fb[j] = (kin && c < ycols) ? Y.get(k, yc0 + c) : czero<R>();
         }
     };
-    fetch(a, b, 0);
-    for (int k0 = 0; k0 < K; k0 += 4) {
-        cx<R> na[MI], nb[NJ];
 #pragma unroll
-        for (int i = 0; i < MI; i++) na[i] = czero<R>();
+    for (int s = 0; s < PF; s++) fetch(a[s], b[s], 4 * s);
+    for (int k0 = 0; k0 < K; k0 += 4 * PF) {
 #pragma unroll
-        for (int j = 0; j < NJ; j++) nb[j] = czero<R>();
-        if (k0 + 4 < K) fetch(na, nb, k0 + 4);
-        // two passes over the blocks, so consecutive products go to
-        // different accumulators (a product waits for the previous one on
-        // its accumulator)
+        for (int s = 0; s < PF; s++) {
+            if (k0 + 4 * s >= K) break;
+            // two passes over the blocks, so consecutive products go to
+            // different accumulators (a product waits for the previous one
+            // on its accumulator)
 #pragma unroll
-        for (int i = 0; i < MI; i++)
+            for (int i = 0; i < MI; i++)
 #pragma unroll
-            for (int j = 0; j < NJ; j++) {
-                acc.re[i][j] = Mfma<R>::mma(a[i].re, b[j].re, acc.re[i][j]);
-                acc.im[i][j] = Mfma<R>::mma(a[i].re, b[j].im, acc.im[i][j]);
-            }
+                for (int j = 0; j < NJ; j++) {
+                    acc.re[i][j] =
+                        Mfma<R>::mma(a[s][i].re, b[s][j].re, acc.re[i][j]);
+                    acc.im[i][j] =
+                        Mfma<R>::mma(a[s][i].re, b[s][j].im, acc.im[i][j]);
+                }
 #pragma unroll
-        for (int i = 0; i < MI; i++)
+            for (int i = 0; i < MI; i++)
 #pragma unroll
-            for (int j = 0; j < NJ; j++) {
-                acc.re[i][j] = Mfma<R>::mma(-a[i].im, b[j].im, acc.re[i][j]);
-                acc.im[i][j] = Mfma<R>::mma(a[i].im, b[j].re, acc.im[i][j]);
-            }
-#pragma unroll
-        for (int i = 0; i < MI; i++) a[i] = na[i];
-#pragma unroll
-        for (int j = 0; j < NJ; j++) b[j] = nb[j];
+                for (int j = 0; j < NJ; j++) {
+                    acc.re[i][j] =
+                        Mfma<R>::mma(-a[s][i].im, b[s][j].im, acc.re[i][j]);
+                    acc.im[i][j] =
+                        Mfma<R>::mma(a[s][i].im, b[s][j].re, acc.im[i][j]);
+                }
+            // the slot takes the step PF ahead
+            fetch(a[s], b[s], k0 + 4 * (s + PF));
+        }
     }
 }
 

@@ -183,14 +183,13 @@ template <typename R, int MI, int NJ> struct CAcc {
 // unrolled by PF so every slot index is a constant): one 4-deep step is
 // only 4 MI NJ products, shorter than an L2 round trip.  FP64 (64 cycles
 // per product) gains from three steps in flight: zposv n = 16384 239 ->
-// 207 ms; FP32 did not (cposv 135 -> 140 ms), so it keeps one.
-template <typename R, int MI, int NJ>
+// 207 ms; the FP32 update did not (cposv 135 -> 140 ms) and keeps one.
+template <typename R, int MI, int NJ, int PF = (sizeof(R) == 8 ? 3 : 1)>
 __device__ __forceinline__ void cprod(CAcc<R, MI, NJ> &acc,
                                       const CView<R> &X, int xr0, int xrows,
                                       const CView<R> &Y, int yc0, int ycols,
                                       int K)
 {
-    constexpr int PF = sizeof(R) == 8 ? 3 : 1;
     if (K <= 0) return;
     const int lane = threadIdx.x & 63;
     const int l16 = lane & 15, lk = lane >> 4;
@@ -336,7 +335,7 @@ __global__ __launch_bounds__(NT) void cjobs_kernel(Dec d, const int *status)
     acc.zero();
     for (int kt = 0; kt < j.nk; kt++) {
         const Operand<R> o = d.operand(j, kt);
-        cprod<R, 2, 2>(acc, o.X, r0, rows, o.Y, c0, cols, o.K);
+        cprod<R, 2, 2, Dec::PF>(acc, o.X, r0, rows, o.Y, c0, cols, o.K);
     }
     cstore<R, 2, 2>(acc, j.C, r0, rows, c0, cols, j.alpha, j.beta,
                     j.lower_only);
@@ -346,6 +345,8 @@ __global__ __launch_bounds__(NT) void cjobs_kernel(Dec d, const int *status)
 // the panel buffer (mb x nb per slot, ld mb): the trsm tasks of
 // pzpotrf.c:47-57 (upper: :90-99), as products with the tile inverse.
 template <typename R> struct PanelDec {
+    // operand steps in flight (cprod): FP64 three, FP32 one
+    static constexpr int PF = (sizeof(R) == 8 ? 3 : 1);
     plasma_desc_t A;
     int upper, k, nbr;
     cx<R> *P;
@@ -383,6 +384,8 @@ template <typename R> struct PanelDec {
 // ncopy jobs copy the panels back into L(m,k); the update reads the
 // buffer, not those tiles.
 template <typename R> struct UpdateDec {
+    // operand steps in flight (cprod): FP64 three, FP32 one
+    static constexpr int PF = (sizeof(R) == 8 ? 3 : 1);
     plasma_desc_t A;
     int upper, k, nbr, ncopy, n_first;
     const cx<R> *P;
@@ -437,6 +440,9 @@ template <typename R> struct UpdateDec {
 // W_kk^H B(k,:) (backward, L^H X = Y) into S (mb x B.n, ld mb): the
 // diagonal trsm tasks of pztrsm.c:113-177, as products with the inverse.
 template <typename R> struct SweepDiagDec {
+    // operand steps in flight (cprod): three (few blocks, K chains
+    // of a whole tile)
+    static constexpr int PF = 3;
     plasma_desc_t B;
     int k, backward, nbr, nbc;
     const cx<R> *W;
@@ -473,6 +479,8 @@ template <typename R> struct SweepDiagDec {
 // L(k,m)^H S for m < k (backward), and the first ncopy jobs write S into
 // B(k,:) (pztrsm.c:113-177, the gemm tasks).
 template <typename R> struct SweepUpdDec {
+    // operand steps in flight (cprod): three (few blocks per launch)
+    static constexpr int PF = 3;
     plasma_desc_t A, B;
     int upper, k, backward, nbr, nbc, ncopy;
     const cx<R> *S;
@@ -527,6 +535,8 @@ template <typename R> struct SweepUpdDec {
 // as Hermitian from its stored triangle (the hemm of zcposv.c:466-472,
 // pzhemm.c Left).
 template <typename R> struct ResidualDec {
+    // operand steps in flight (cprod): three
+    static constexpr int PF = 3;
     plasma_desc_t A, X, Rd;
     int upper, nbr, nbc;
 

@@ -156,17 +156,17 @@ __global__ __launch_bounds__(NTHREADS) void amax_cols_kernel(
 // R(m, tn) -= sum_k Asym(m, k) X(k, tn) with Asym(m,k) = L(m,k) (k < m),
 // L(k,m)^T (k > m), and the diagonal tile read symmetrically from its
 // stored triangle.  64 x 64 output blocks; K runs over every tile column in
-// 16-deep chunks through double-buffered LDS with 16-byte loads (33 KB of
-// LDS with the padded rows: up to four workgroups share a CU and hide each
-// other's loads); MFMA
+// 16-deep (FP64) or 32-deep (FP32) chunks through double-buffered LDS with
+// 16-byte loads (33 KB of LDS with the padded rows: up to four workgroups
+// share a CU and hide each other's loads); MFMA
 // accumulation in the precision of A.  The product is formed transposed
 // (X^T Asym^T) so the accumulator's consecutive lanes are consecutive rows
 // of R.
 // ---------------------------------------------------------------------------
-constexpr int SY_BM = 64, SY_BN = 64, SY_KC = 16;
+constexpr int SY_BM = 64, SY_BN = 64;
 constexpr int SY_WM = 32, SY_WN = 32;   // per wave: right-hand sides x rows
 
-template <typename T>
+template <typename T, int SY_KC>
 __global__ __launch_bounds__(NTHREADS) void symm_residual_kernel(
     plasma_desc_t A, plasma_desc_t X, plasma_desc_t R, int upper,
     int rchunks, int cchunks, int ksplit, T *part, const int *status)
@@ -357,7 +357,13 @@ int launch_symm(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t X,
     if (work == nullptr || R.i % R.mb != 0 || R.j % R.nb != 0) ksplit = 1;
     else ksplit = (int)min((size_t)ksplit, work_bytes / part_bytes);
     if (ksplit < 1) ksplit = 1;
-    hipLaunchKernelGGL(symm_residual_kernel<T>,
+    // chunk depth: 32 in FP32 (half the MFMA work per chunk of FP64, so
+    // a deeper chunk hides the load and barrier), 16 in FP64.  Measured on
+    // the Cholesky adjoint's 16384 x 16384 symmetric product: FP32 63 ms
+    // (16) -> 56 ms (32) -> 67 ms (64); FP64 99 ms (16), 102 ms (32),
+    // 121 ms (64).
+    constexpr int KC = sizeof(T) == 4 ? 32 : 16;
+    hipLaunchKernelGGL((symm_residual_kernel<T, KC>),
                        dim3(R.mt * rchunks, R.nt * cchunks, ksplit),
                        dim3(NTHREADS), 0, s, A, X, R, uplo == PlasmaUpper,
                        rchunks, cchunks, ksplit, (T *)work, status);

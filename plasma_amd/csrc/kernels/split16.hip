@@ -36,7 +36,11 @@
 // registers, then the loads of the next chunk start into the other stage
 // and run under the chunk's products; one barrier per chunk.  The waits
 // are explicit around bare s_barrier instructions (a __syncthreads()
-// fence would also wait for stores in flight).
+// fence would also wait for stores in flight).  C is scaled by exact
+// reciprocals of the row scales (no division); an interior block of the
+// lower triangle is staged through LDS and written back with 16-byte
+// non-temporal stores (a quarter of the store instructions of the
+// accumulator layout, whose lanes hold four columns each).
 #include "common.hip.h"
 #include "tilegemm.hip.h"
 #include "plasma_kernels.h"
@@ -51,6 +55,8 @@ constexpr int NT = 256;
 constexpr int BM = 128;
 constexpr int KC = 32;
 constexpr int FRAG = 512;           // fp16 elements per fragment
+constexpr int SST = 68;             // column stride of the C staging area
+constexpr int STAGE_BYTES = 4 * 64 * SST * 4;   // four waves' 64 x 64 blocks
 
 struct Geom {
     int rbp;                         // row blocks per tile (multiple of 8)
@@ -325,9 +331,50 @@ __device__ __forceinline__ void run16(const Update16 &u, int tx, int ty,
                 acc[i][j] = v;
             }
     }
-    // every wave's last reads are done before LDS is reused by the next job
+    // every wave's last reads of the stages are done before LDS is reused
     wait_lgkm0();
     bar();
+    // exact reciprocals of the power-of-two scales: 2^k = 0.5 * 2^(k+1)
+    // has frexp exponent k + 1
+    float ix[4][4], iy[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            ix[i][q] = __builtin_ldexpf(
+                1.0f, 1 - __builtin_amdgcn_frexp_expf(fx[i][q]));
+            iy[i][q] = __builtin_ldexpf(
+                1.0f, 1 - __builtin_amdgcn_frexp_expf(fy[i][q]));
+        }
+    if (CROW && !lower && xrows >= BM && yrows >= BM && C.rs == 1 &&
+        (C.cs & 3) == 0 && (((uintptr_t)C.p) & 15) == 0) {
+        // Interior block: each wave stages its 64 x 64 result in its own
+        // LDS region (column stride SST floats) and writes 16-byte row
+        // quads, four columns of 256 contiguous bytes per store, marked
+        // non-temporal (C is not read again in this launch).
+        float *w = reinterpret_cast<float *>(lds) + wave * 64 * SST;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    w[(j * 16 + 4 * (lane / 16) + q) * SST + i * 16 +
+                      lane % 16] = -acc[i][j][q] * ix[i][q] * iy[j][q];
+        wait_lgkm0();   // the region is the wave's own: no barrier
+        float *cb = C.p + wr * 64 + (ptrdiff_t)(wc * 64) * C.cs;
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+            const int cc = 4 * t + lane / 16, rr = 4 * (lane % 16);
+            const f32x4 v = *reinterpret_cast<const f32x4 *>(w + cc * SST +
+                                                             rr);
+            __builtin_nontemporal_store(
+                v, reinterpret_cast<f32x4 *>(cb + (ptrdiff_t)cc * C.cs + rr));
+        }
+        // the next job's operand loads write LDS
+        bar();
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -336,7 +383,7 @@ __device__ __forceinline__ void run16(const Update16 &u, int tx, int ty,
             for (int q = 0; q < 4; q++) {
                 const int r = crow(i, q), c = ccol(j, q);
                 if (r < xrows && c < yrows && (!lower || c + diag_off <= r))
-                    C.at(r, c) = -acc[i][j][q] / (fx[i][q] * fy[j][q]);
+                    C.at(r, c) = -acc[i][j][q] * ix[i][q] * iy[j][q];
             }
 }
 
@@ -344,8 +391,10 @@ template <bool CROW, int NP>
 __global__ __launch_bounds__(NT, 2) void split16_update_kernel(
     Update16 u, int njobs, const int *status)
 {
-    __shared__ __attribute__((aligned(1024))) _Float16 lds[2 * 2 * NP * 8 *
-                                                           FRAG];
+    // two operand stages, reused as the C staging area after the products
+    constexpr int OPS = 2 * 2 * NP * 8 * FRAG, STG = STAGE_BYTES / 2;
+    __shared__ __attribute__((aligned(1024))) _Float16 lds[OPS > STG ? OPS
+                                                                     : STG];
     if (block_should_stop(status)) return;
     const int padded = tg::xcd_grid(njobs), chunk = padded / tg::NXCD;
     for (int b = blockIdx.x; b < padded; b += gridDim.x) {

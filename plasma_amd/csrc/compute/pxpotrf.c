@@ -370,13 +370,15 @@ static int pxpotrf_launch(plasma_enum_t uplo, plasma_desc_t A, void *inv,
     plasma_context_t *ctx = plasma_context_self();
     const int gw = imax(1, imin(ctx->group, A.mt));
     /* Blocked panel solve (diagonal-block inverses only on the critical
-     * path): FP32 without lookahead, where it shortens the serial chain
-     * (n = 32768: diagonal tiles 36.4 -> 24.6 ms, panels 6.5 -> 13.1 ms,
-     * factor 152.3 -> 147.4 ms).  Beside a running update its panel
-     * slows more than the diagonal tile gains (FP32 with lookahead: no
-     * gain; FP64: 252.5 -> 281.8 ms), so the other schedules keep the
-     * full-inverse panel GEMM. */
-    const int blocked = A.precision == PlasmaRealFloat && !lookahead &&
+     * path) in FP32, where it shortens the serial chain.  Without
+     * lookahead (n = 32768): diagonal tiles 36.4 -> 24.6 ms, panels 6.5 ->
+     * 13.1 ms, factor 152.3 -> 147.4 ms.  With lookahead 1 and the fp16
+     * split update, the chain of diagonal tile, panel and next column is
+     * the critical path of the last half of the factorization: diagonal
+     * tiles 27.6 -> 17.9 ms, panels 14.9 -> 26.6 ms, dsposv 98.6 -> 97.8
+     * ms.  In FP64 its panel slows more than the diagonal tile gains
+     * (252.5 -> 281.8 ms), so FP64 keeps the full-inverse panel GEMM. */
+    const int blocked = A.precision == PlasmaRealFloat &&
                         plasma_panel_blocked_ok(A);
     const int split = use_split(A);
     const int pcols = gw * A.nb;

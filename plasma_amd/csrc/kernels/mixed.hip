@@ -155,23 +155,25 @@ __global__ __launch_bounds__(NTHREADS) void amax_cols_kernel(
 // ---------------------------------------------------------------------------
 // R(m, tn) -= sum_k Asym(m, k) X(k, tn) with Asym(m,k) = L(m,k) (k < m),
 // L(k,m)^T (k > m), and the diagonal tile read symmetrically from its
-// stored triangle.  64 x 64 output blocks; K runs over every tile column in
+// stored triangle.  Output blocks of 64 right-hand sides x 64 (FP64) or
+// 128 (FP32) rows; K runs over every tile column in
 // 16-deep (FP64) or 32-deep (FP32) chunks through double-buffered LDS with
-// 16-byte loads (33 KB of LDS with the padded rows: up to four workgroups
-// share a CU and hide each other's loads); MFMA
+// 16-byte loads (with the padded rows 33 KB of LDS in FP64, 51 KB in FP32:
+// four or three workgroups share a CU and hide each other's loads); MFMA
 // accumulation in the precision of A.  The product is formed transposed
 // (X^T Asym^T) so the accumulator's consecutive lanes are consecutive rows
 // of R.
 // ---------------------------------------------------------------------------
-constexpr int SY_BM = 64, SY_BN = 64;
-constexpr int SY_WM = 32, SY_WN = 32;   // per wave: right-hand sides x rows
+constexpr int SY_BN = 64;
+constexpr int SY_WM = 32;               // per wave: right-hand sides
 
-template <typename T, int SY_KC>
+template <typename T, int SY_KC, int SY_BM>
 __global__ __launch_bounds__(NTHREADS) void symm_residual_kernel(
     plasma_desc_t A, plasma_desc_t X, plasma_desc_t R, int upper,
     int rchunks, int cchunks, int ksplit, T *part, const int *status)
 {
-    // first operand: 64 right-hand sides; second: 64 rows of Asym
+    // first operand: 64 right-hand sides; second: SY_BM rows of Asym
+    constexpr int SY_WN = SY_BM / 2;    // per wave: rows
     typedef GemmNT<T, SY_BN, SY_BM, SY_KC, SY_WM, SY_WN> G;
     __shared__ typename G::Smem sm;
     if (block_should_stop(status)) return;
@@ -337,16 +339,16 @@ __global__ __launch_bounds__(NTHREADS) void symm_reduce_kernel(
     C.at(row % R.mb, col % R.nb) -= sum;
 }
 
-// The grid has one workgroup per 64 x 64 block of R (column blocks only
+// The grid has one workgroup per block of R (column blocks only
 // over the right-hand sides present: min(nb, n) columns per tile column);
 // when that is small (few right-hand sides) and work has room, K is split
 // so that about SY_TARGET workgroups share the chip.
 constexpr int SY_TARGET = 4096, SY_MAXSPLIT = 8;
 
-template <typename T>
-int launch_symm(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t X,
-                plasma_desc_t R, void *work, size_t work_bytes,
-                const int *status, hipStream_t s)
+template <typename T, int SY_BM>
+int launch_symm_rb(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t X,
+                   plasma_desc_t R, void *work, size_t work_bytes,
+                   const int *status, hipStream_t s)
 {
     const int rchunks = (R.mb + SY_BM - 1) / SY_BM;
     const int cchunks = (min(R.nb, R.n) + SY_BN - 1) / SY_BN;
@@ -363,7 +365,7 @@ int launch_symm(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t X,
     // (16) -> 56 ms (32) -> 67 ms (64); FP64 99 ms (16), 102 ms (32),
     // 121 ms (64).
     constexpr int KC = sizeof(T) == 4 ? 32 : 16;
-    hipLaunchKernelGGL((symm_residual_kernel<T, KC>),
+    hipLaunchKernelGGL((symm_residual_kernel<T, KC, SY_BM>),
                        dim3(R.mt * rchunks, R.nt * cchunks, ksplit),
                        dim3(NTHREADS), 0, s, A, X, R, uplo == PlasmaUpper,
                        rchunks, cchunks, ksplit, (T *)work, status);
@@ -375,6 +377,20 @@ int launch_symm(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t X,
                            (const T *)work, status);
     }
     return hipGetLastError() == hipSuccess ? 0 : PlasmaErrorInternal;
+}
+
+// Rows of R per workgroup: 128 in FP32 (every chunk of X then feeds twice
+// the rows), 64 in FP64.  Measured on the Cholesky adjoint's 16384 x 16384
+// symmetric product: FP32 56.1 -> 52.6 ms with 128 rows; FP64 98.6 ->
+// 101.2 ms; the dsposv residual (n = 32768, 64 right-hand sides) is the
+// same with either.
+template <typename T>
+int launch_symm(plasma_enum_t uplo, plasma_desc_t A, plasma_desc_t X,
+                plasma_desc_t R, void *work, size_t work_bytes,
+                const int *status, hipStream_t s)
+{
+    constexpr int RB = sizeof(T) == 4 ? 128 : 64;
+    return launch_symm_rb<T, RB>(uplo, A, X, R, work, work_bytes, status, s);
 }
 
 }  // namespace

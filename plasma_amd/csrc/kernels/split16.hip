@@ -38,8 +38,8 @@
 // are explicit around bare s_barrier instructions (a __syncthreads()
 // fence would also wait for stores in flight).  C is scaled by exact
 // reciprocals of the row scales (no division); an interior block of the
-// lower triangle is staged through LDS and written back with 16-byte
-// non-temporal stores (a quarter of the store instructions of the
+// lower triangle is read and written through LDS with 16-byte loads and
+// non-temporal stores (a quarter of the memory instructions of the
 // accumulator layout, whose lanes hold four columns each).
 #include "common.hip.h"
 #include "tilegemm.hip.h"
@@ -234,18 +234,58 @@ __device__ __forceinline__ void run16(const Update16 &u, int tx, int ty,
     };
 
     // C block and the row / column scales of this lane's elements.  The
-    // raw loads come first (masked, no arithmetic between them, so they
-    // issue back to back); the scaling follows once they are all in.
+    // accumulator layout gives each lane four C columns.  An interior block
+    // of the lower triangle goes through LDS: each wave reads its 64 x 64
+    // part as 16-byte row quads (four columns of 256 contiguous bytes per
+    // load) into its own region (column stride SST), then its elements in
+    // the accumulator layout; the operand stages reuse that LDS after a
+    // barrier.  Other blocks load element by element (masked, no
+    // arithmetic between the loads, so they issue back to back).  The
+    // scaling follows once they are all in.
+    const bool staged = CROW && !lower && xrows >= BM && yrows >= BM &&
+                        C.rs == 1 && (C.cs & 3) == 0 &&
+                        (((uintptr_t)C.p) & 15) == 0;
     f32x4 acc[4][4];
+    if (staged) {
+        float *w = reinterpret_cast<float *>(lds) + wave * 64 * SST;
+        const float *cb = C.p + wr * 64 + (ptrdiff_t)(wc * 64) * C.cs;
+        f32x4 quad[16];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+        for (int t = 0; t < 16; t++) {
+            const int cc = 4 * t + lane / 16, rr = 4 * (lane % 16);
+            quad[t] = *reinterpret_cast<const f32x4 *>(cb +
+                                                       (ptrdiff_t)cc * C.cs +
+                                                       rr);
+        }
 #pragma unroll
-        for (int j = 0; j < 4; j++)
+        for (int t = 0; t < 16; t++) {
+            const int cc = 4 * t + lane / 16, rr = 4 * (lane % 16);
+            *reinterpret_cast<f32x4 *>(w + cc * SST + rr) = quad[t];
+        }
+        wait_lgkm0();   // the region is the wave's own: no barrier
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int r = crow(i, q), c = ccol(j, q);
-                acc[i][j][q] = (r < xrows && c < yrows) ? C.at(r, c) : 0.0f;
-            }
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    acc[i][j][q] = w[(j * 16 + 4 * (lane / 16) + q) * SST +
+                                     i * 16 + lane % 16];
+        wait_lgkm0();
+        bar();          // every wave has read its region: stages reusable
+    }
+    else {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int r = crow(i, q), c = ccol(j, q);
+                    acc[i][j][q] = (r < xrows && c < yrows) ? C.at(r, c)
+                                                            : 0.0f;
+                }
+    }
     // scale of C element (i, j, q): sx_[crow] * sy_[ccol], indices clamped
     // to the valid rows (the clamped values are never used)
     float fx[4][4], fy[4][4];
@@ -346,8 +386,7 @@ __device__ __forceinline__ void run16(const Update16 &u, int tx, int ty,
             iy[i][q] = __builtin_ldexpf(
                 1.0f, 1 - __builtin_amdgcn_frexp_expf(fy[i][q]));
         }
-    if (CROW && !lower && xrows >= BM && yrows >= BM && C.rs == 1 &&
-        (C.cs & 3) == 0 && (((uintptr_t)C.p) & 15) == 0) {
+    if (staged) {
         // Interior block: each wave stages its 64 x 64 result in its own
         // LDS region (column stride SST floats) and writes 16-byte row
         // quads, four columns of 256 contiguous bytes per store, marked

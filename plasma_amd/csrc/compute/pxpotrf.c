@@ -313,15 +313,13 @@ static int pxpotrf_groups(const group_t *gp, int depth, void *panel[],
 
     hipEvent_t ev_fork = ev[0], ev_join = ev[1];
     hipEvent_t *ev_factor = ev + 2, *ev_bulk = ev + 2 + EV_RING;
-    /* The bulk update's grid: in FP32 (two update workgroups per CU) it is
-     * capped to leave 64 workgroup slots to the lookahead stream, which
-     * then starts its diagonal factorization and panel solve without
-     * waiting for update workgroups to drain (n = 32768: 157.6 -> 148.1
-     * ms).  In FP64 the cap costs the update more than the lookahead
-     * gains (249.5 -> 262.9 ms), so the grid is uncapped (0). */
-    const int bulk_wgs = (A.precision == PlasmaRealFloat)
-                             ? imax(2 * ctx->num_cus - 64, ctx->num_cus)
-                             : 0;
+    /* The bulk update's grid: one workgroup per job (0: uncapped), so that
+     * the hardware dispatcher hands the lookahead stream's workgroups the
+     * slots of finishing jobs.  A persistent grid capped below the chip's
+     * slots was measured against it: FP32 dsposv n = 32768, cond 1e6, 2
+     * CUs - 64 workgroups 97.3 ms, 2 CUs - 128 96.8 ms, 2 CUs 101.8 ms,
+     * uncapped 92.6 ms; FP64 capped 262.9 ms, uncapped 249.5 ms. */
+    const int bulk_wgs = 0;
 
     /* fork: the panel stream starts after everything queued so far */
     hipEventRecord(ev_fork, s0);
@@ -376,8 +374,9 @@ static int pxpotrf_launch(plasma_enum_t uplo, plasma_desc_t A, void *inv,
      * split update, the chain of diagonal tile, panel and next column is
      * the critical path of the last half of the factorization: diagonal
      * tiles 27.6 -> 17.9 ms, panels 14.9 -> 26.6 ms, dsposv 98.6 -> 97.8
-     * ms.  In FP64 its panel slows more than the diagonal tile gains
-     * (252.5 -> 281.8 ms), so FP64 keeps the full-inverse panel GEMM. */
+     * ms (with the uncapped bulk grid: 93.6 -> 92.5 ms).  In FP64 its
+     * panel slows more than the diagonal tile gains (252.5 -> 281.8 ms),
+     * so FP64 keeps the full-inverse panel GEMM. */
     const int blocked = A.precision == PlasmaRealFloat &&
                         plasma_panel_blocked_ok(A);
     const int split = use_split(A);

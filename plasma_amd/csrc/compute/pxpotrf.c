@@ -596,8 +596,14 @@ void plasma_pxpotrf(plasma_enum_t uplo, plasma_desc_t A, void *inv,
     int *st = plasma_sequence_dstatus(sequence, request);
     hipStream_t s0 = ctx->stream;
     const int gw = imax(1, imin(ctx->group, A.mt));
-    /* lookahead depth: at most the number of groups after the first */
-    const int lookahead = imin(ctx->lookahead, (A.mt + gw - 1) / gw - 1);
+    /* lookahead depth: at most the number of groups after the first.  FP32
+     * factorizations run at most 2: a tuning run crashed the host process
+     * (SIGSEGV) in an FP32 factorization with lookahead 3 (n = 32768,
+     * update depth 4, graph mode) and the cause is not found; depth 3 never
+     * beat depth 1 in any measurement. */
+    const int la_max = A.precision == PlasmaRealFloat ? 2 : ctx->lookahead;
+    const int lookahead = imin(imin(ctx->lookahead, la_max),
+                               (A.mt + gw - 1) / gw - 1);
 
     if (ctx->graph_capture == PlasmaEnabled && !ctx->profile) {
         plasma_graph_entry_t *e = graph_get(uplo, A, inv, lookahead,

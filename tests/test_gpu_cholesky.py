@@ -182,10 +182,10 @@ def test_spotrf_fp16_split_badly_scaled(plasma, nb_guard, uplo, lookahead):
 @pytest.mark.parametrize("lookahead", [0, 1, 3])
 @pytest.mark.parametrize("n,nb", [(1000, 128), (1300, 256)])
 def test_spotrf_panel_schedules(plasma, nb_guard, uplo, lookahead, n, nb):
-    """FP32 without lookahead solves its panels by 32-column block
-    substitution (diagonal-block inverses only); with lookahead by the
-    full-inverse GEMM: both against the oracle, and sposv's residual (the
-    sweeps need the whole W_kk, formed after the factorization)."""
+    """FP32 solves its panels by 32-column block substitution
+    (diagonal-block inverses only), with and without lookahead (FP32
+    clamps lookahead 3 to 2): against the oracle, and sposv's residual
+    (the sweeps need the whole W_kk, formed after the factorization)."""
     pa.set_param(pa.PlasmaNb, nb)
     pa.set_param(pa.PlasmaLookahead, lookahead)
     A = U.ggt_spd(n, seed=n + lookahead, dtype=np.float32)

@@ -66,6 +66,43 @@ __global__ __launch_bounds__(NTHREADS) void lansy_tile_kernel(
         }
         return;
     }
+    if (L.rs == 1 && mv <= 256) {
+        // One pass over a column-major tile: wave w takes the columns
+        // c = w, w + 4, ...; lane l the rows l, l + 64, ... (consecutive
+        // lanes read consecutive rows).  A column's sum is a wave
+        // reduction; a row's sum adds the four waves' partial sums in a
+        // fixed order.
+        __shared__ double part[NTHREADS / 64][256];
+        const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+        double rsum[4] = {0, 0, 0, 0};
+        double *csum = work + m * ld + (size_t)n * A.mb;
+#pragma unroll 2
+        for (int c = wave; c < nv; c += NTHREADS / 64) {
+            const T *col = L.p + (ptrdiff_t)c * L.cs;
+            double s = 0;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int r = lane + 64 * u;
+                const double x = r < mv ? fabs((double)col[r]) : 0.0;
+                rsum[u] += x;
+                s += x;
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+            if (lane == 0) csum[c] = s;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) part[wave][lane + 64 * u] = rsum[u];
+        __syncthreads();
+        double *rout = work + n * ld + (size_t)m * A.mb;
+        for (int r = t; r < mv; r += NTHREADS) {
+            double s = part[0][r];
+#pragma unroll
+            for (int w = 1; w < NTHREADS / 64; w++) s += part[w][r];
+            rout[r] = s;
+        }
+        return;
+    }
     abs_row_sums<T>(L, mv, nv, work + n * ld + (size_t)m * A.mb);
     View<T> Lt;
     Lt.p = L.p;
@@ -134,13 +171,23 @@ __global__ __launch_bounds__(NTHREADS) void amax_cols_kernel(
     const int tn = j / A.nb, c = j % A.nb;
     __shared__ double red[NTHREADS];
     double v = 0;
-    for (int tm = 0; tm < A.mt; tm++) {
-        const View<T> V = tile_view<T>(A, tm, tn);
-        const int mv = plasma_tile_mview(A, tm);
-        for (int r = threadIdx.x; r < mv; r += NTHREADS) {
-            double a = fabs((double)V.at(r, c));
-            if (a > v) v = a;
+    // rows of the column as one index i = tm mb + r: every thread takes
+    // rows i = t, t + NTHREADS, ..., four passes at a time so that their
+    // loads are in flight together
+    const int rows = A.mt * A.mb;
+    for (int i0 = threadIdx.x; i0 < rows; i0 += 4 * NTHREADS) {
+        double a[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int i = i0 + u * NTHREADS;
+            const int tm = i / A.mb, r = i - tm * A.mb;
+            a[u] = 0;
+            if (i < rows && r < plasma_tile_mview(A, tm))
+                a[u] = fabs((double)tile_view<T>(A, tm, tn).at(r, c));
         }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (a[u] > v) v = a[u];
     }
     red[threadIdx.x] = v;
     __syncthreads();

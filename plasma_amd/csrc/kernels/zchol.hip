@@ -778,6 +778,39 @@ __global__ __launch_bounds__(NT) void zlanhe_tile_kernel(plasma_desc_t A,
     CView<R> L = lower_cview<R>(A, upper != 0, m, n);
     const size_t ld = (size_t)A.mt * A.mb;
     if (m == n) L.herm = 1;
+    if (m != n && mv <= 256) {
+        // One pass over an off-diagonal tile: wave w takes the columns
+        // c = w, w + 4, ...; lane l the rows l, l + 64, ... (consecutive
+        // rows of a column-major tile to consecutive lanes).  A column's
+        // sum is a wave reduction; a row's sum adds the four waves' partial
+        // sums in a fixed order.
+        __shared__ double part[NT / 64][256];
+        const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+        double rsum[4] = {0, 0, 0, 0};
+        for (int c = wave; c < nv; c += NT / 64) {
+            double s = 0;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int r = lane + 64 * u;
+                const double x = r < mv ? cabs_of(L.get(r, c)) : 0.0;
+                rsum[u] += x;
+                s += x;
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+            if (lane == 0) work[m * ld + (size_t)n * A.mb + c] = s;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) part[wave][lane + 64 * u] = rsum[u];
+        __syncthreads();
+        for (int r = t; r < mv; r += NT) {
+            double s = part[0][r];
+#pragma unroll
+            for (int w = 1; w < NT / 64; w++) s += part[w][r];
+            work[n * ld + (size_t)m * A.mb + r] = s;
+        }
+        return;
+    }
     for (int r = threadIdx.x; r < mv; r += NT) {
         double s = 0;
         for (int c = 0; c < nv; c++) s += cabs_of(L.get(r, c));
@@ -803,14 +836,25 @@ __global__ __launch_bounds__(NT) void zamax_cols_kernel(plasma_desc_t A,
     const int j = blockIdx.x;
     const int tn = j / A.nb, c = j % A.nb;
     double v = 0;
-    for (int tm = 0; tm < A.mt; tm++) {
-        const CView<R> V = tile_cview<R>(A, tm, tn);
-        const int mv = plasma_tile_mview(A, tm);
-        for (int r = threadIdx.x; r < mv; r += NT) {
-            const cx<R> z = V.get(r, c);
-            const double a = fabs((double)z.re) + fabs((double)z.im);
-            if (a > v) v = a;
+    // rows of the column as one index i = tm mb + r: every thread takes
+    // rows i = t, t + NT, ..., four passes at a time so that their loads
+    // are in flight together
+    const int rows = A.mt * A.mb;
+    for (int i0 = threadIdx.x; i0 < rows; i0 += 4 * NT) {
+        double a[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int i = i0 + u * NT;
+            const int tm = i / A.mb, r = i - tm * A.mb;
+            a[u] = 0;
+            if (i < rows && r < plasma_tile_mview(A, tm)) {
+                const cx<R> z = tile_cview<R>(A, tm, tn).get(r, c);
+                a[u] = fabs((double)z.re) + fabs((double)z.im);
+            }
         }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (a[u] > v) v = a[u];
     }
     red[threadIdx.x] = v;
     __syncthreads();
